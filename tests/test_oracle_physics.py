@@ -1,0 +1,156 @@
+"""Closed-form and invariant checks of the oracle's hot-path functions that
+no reference-held vector covers: QDHT (symmetry, involution, Gaussian
+Hankel pair), plasma recurrence limits, W(I) table behaviour, the Kerr/Raman
+response and the RK4 combine.  CPU only, small grids."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN
+
+
+def make_cfg(path, **kw):
+    base = dict(MAX_TOLERANCE="1e-3", LAMBDA_V="0.8e-6", N_T="256", T_MIN="-100e-15", T_MAX="100e-15",
+                DISPERSION_FILE="SiO2.rawdisp", N_R="64", R_MAX="0.3e-3", N_Z="3", Z_MIN="0", Z_MAX="2e-4",
+                ZNET_TYPE="e", PULSE_SHAPE="gg", DURATION_FWHM="30e-15", DIAMETER_FWHM="8e-5",
+                WAVEFRONT_RADIUS="1e30", ENERGY="1e-6", NONLIN_REFRINDEX="2.5e-20", RAMAN_FRACTION="0.18",
+                TAU_RAMAN="32e-15", OMEGA_RAMAN="8.2938e13", NEUTRAL_DENSITY="2.1e28", RECOMBINATION_TAU="150e-15",
+                AMBIENT_CARRIERS="0", COLLISION_TAU="23e-15", IONIZATION_POTENTIAL="9", DUMPS_N="0")
+    base.update({k: str(v) for k, v in kw.items()})
+    import shutil
+    d = os.path.dirname(path)
+    if not os.path.exists(os.path.join(d, "SiO2.rawdisp")):
+        shutil.copyfile(os.path.join(GOLDEN, "SiO2.rawdisp"), os.path.join(d, "SiO2.rawdisp"))
+    with open(path, "w") as f:
+        for k, v in base.items():
+            f.write("%s %s\n" % (k, v))
+    return path
+
+
+@pytest.fixture
+def small(tmp_path):
+    return oracle.Sim(make_cfg(str(tmp_path / "c.txt")), nthreads=4)
+
+
+def test_qdht_matrix_properties(small):
+    C = small.get("qdht_c")
+    assert np.abs(C - C.T).max() < 1e-15
+    assert np.abs(C @ C - np.eye(small.R)).max() < 1e-8   # discretisation limit at N=64
+
+
+def test_qdht_gaussian_pair(tmp_path):
+    """Y_i = m1_i sum_j C_ij X_j / m1_j samples (V/R) g(nu_i), V = S/(2 pi R) the
+    band limit, nu_i = j_i/(2 pi R), g(nu) = 2 pi int f(r) J0(2 pi nu r) r dr; for
+    f = exp(-r^2/a^2), g = pi a^2 exp(-pi^2 a^2 nu^2)."""
+    sim = oracle.Sim(make_cfg(str(tmp_path / "c.txt"), N_R=256, N_T=8), nthreads=2)
+    R = sim.scalar("XMAX")
+    S = sim.scalar("qS")
+    x = sim.get("qdht_x")
+    r = x * R
+    a = R / 8
+    X = np.zeros((sim.R, sim.T), complex)
+    X[:, :] = np.exp(-(r / a) ** 2)[:, None]
+    Y = sim.qdht(X)
+    nu = x * S / (2 * np.pi * R)
+    g = np.pi * a * a * np.exp(-(np.pi * a * nu) ** 2) * (S / (2 * np.pi * R)) / R
+    assert np.abs(Y[:, 3] - g).max() / g.max() < 1e-12
+    # involution: applying it twice returns the input
+    assert np.abs(sim.qdht(Y) - X).max() < 1e-12
+
+
+def test_plasma_pure_recombination_decay(tmp_path):
+    """E = 0: W = 0 and avalanche = 0, Heun on rho' = -rho/tau_r gives
+    rho_n = rho_0 (1 - h + h^2/2)^n with h = dt/tau_r."""
+    sim = oracle.Sim(make_cfg(str(tmp_path / "c.txt"), AMBIENT_CARRIERS="1e-3"), nthreads=1)
+    rho, loss = sim.plasma(np.zeros(sim.T, complex))
+    h = sim.scalar("TSTEP") * 1e20 / sim.scalar("RECOMBINATION_TAU")
+    n = np.arange(sim.T)
+    assert rho[0] > 0
+    np.testing.assert_allclose(rho, rho[0] * (1 - h + h * h / 2) ** n, rtol=1e-13)
+    assert np.all(loss == 0)
+
+
+def test_plasma_threshold_and_saturation(small):
+    T = small.T
+    # below 1e13 W/m^2 (1e-3 in solver units) nothing ionizes
+    e = np.full(T, np.sqrt(0.99e-3) + 0j)
+    rho, loss = small.plasma(e)
+    assert np.all(rho == 0) and np.all(loss == 0)
+    # the first interval starts from E = 0 (ionization.cpp:25): rho_1 only sees W(E_1)/2
+    e = np.full(T, np.sqrt(50.0) + 0j)          # 5e17 W/m^2
+    rho, loss = small.plasma(e)
+    W = small.photoionization(np.sqrt(50.0), 0.0)
+    dt = small.scalar("TSTEP") * 1e20
+    assert abs(rho[1] - 0.5 * dt * W) <= 1e-12 * rho[1]
+    assert np.all(np.diff(rho) >= 0)
+    # loss gating compares the solver-unit intensity with the ln threshold 29.93 (quirk)
+    assert loss[0] == 0 and loss[T - 1] == 0 and np.all(loss[1:T - 1] != 0)
+    e2 = np.full(T, np.sqrt(29.0) + 0j)
+    assert np.all(small.plasma(e2)[1] == 0)
+    # rho is clamped at the neutral density
+    big = np.full(T, np.sqrt(5e4) + 0j)          # 5e20 W/m^2
+    rho, _ = small.plasma(big)
+    assert rho.max() <= small.scalar("NEUTRAL_DENSITY")
+
+
+def test_ionization_multiphoton_slope(small):
+    """gamma >> 1: ln W ~ K ln I with K = floor(U / hbar w0 + 1) = 6 for 9 eV at 800 nm."""
+    l1, l2 = 30.0, 30.5
+    slope = (small.keldysh_rate_ln(l2) - small.keldysh_rate_ln(l1)) / (l2 - l1)
+    assert abs(slope - 6) < 0.3
+    tab = small.get("ionrate_ln")
+    assert tab[8191] == 0.0 and np.all(np.diff(tab[:2000]) > 0)   # monotone in the multiphoton range
+    # table entry i is the rate at lnI = 29.9336 + 0.003304 i, minus ln(1e20)
+    i = 1234
+    assert abs(tab[i] - (small.keldysh_rate_ln(29.933606208922594 + 0.003304003148325 * i) - 46.0517018)) < 1e-12
+
+
+def test_photoionization_blast_error(small):
+    with pytest.raises(oracle.OracleError, match="blast"):
+        small.photoionization(np.sqrt(1e12), 0.0)
+
+
+def test_kerr_raman_response_low_intensity(small):
+    """Below the ionization threshold the RHS is the Kerr/Raman term only:
+    N = -i K(w) FFT(y E), y = (1-fR) I + R (*) I (circular), E = IFFT(A)/T."""
+    T = small.T
+    rng = np.random.default_rng(5)
+    t = np.linspace(-1, 1, T)
+    E = 0.02 * np.exp(-(t / 0.2) ** 2) * np.exp(1j * rng.uniform(0, 0.3, T))
+    A = np.fft.fft(E)
+    N = small.nl_response(A)
+    Iv = np.abs(E) ** 2
+    dt = small.scalar("TSTEP")
+    fr, tau, Om = small.scalar("RAMAN_FRACTION"), small.scalar("TAU_RAMAN"), small.scalar("OMEGA_RAMAN")
+    M = dt * (1 + Om**2 * tau**2) / tau / (Om * tau)
+    z = np.exp((-1 / tau + 1j * Om) * dt)
+    s = np.zeros(T, complex)
+    acc = 0
+    for n in range(T):
+        acc = z * acc + Iv[n]
+        s[n] = acc
+    y = (1 - fr) * Iv + fr * M * (s * (1 - z**T) + z ** (np.arange(T) + 1) * s[-1]).imag
+    ref = -1j * small.get("kerr") * np.fft.fft(y * E)
+    assert np.abs(N - ref).max() / np.abs(ref).max() < 1e-12
+
+
+def test_rk4_combine(small):
+    """B1 = (k1 + 2 k2 + 2 k3 + k4)/6 with stages at A, A + dz/2 k1, A + dz/2 k2,
+    A + dz k3 (classical RK4), and the two max-norms over spectral samples."""
+    F = small.get("field")
+    dz = 3e-6
+    small.set_scalar("ZSTEP", dz)
+    maxA2, maxNL2 = small.nl_step()
+    B1 = small.get("b1")
+    r = 5
+    a = F[r]
+    k1 = small.nl_response(a)
+    k2 = small.nl_response(a + dz / 2 * k1)
+    k3 = small.nl_response(a + dz / 2 * k2)
+    k4 = small.nl_response(a + dz * k3)
+    ref = (k1 + 2 * k2 + 2 * k3 + k4) / 6
+    assert np.abs(B1[r] - ref).max() <= 1e-13 * np.abs(ref).max()
+    assert maxA2 == np.max(np.abs(F) ** 2)
+    assert abs(maxNL2 - np.max(np.abs(B1) ** 2)) <= 1e-15 * maxNL2
