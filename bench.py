@@ -1,0 +1,156 @@
+"""Benchmark of the gsolver_solid_qdht z-march on one MI355X.
+
+Workload: the production grid N_r = 1024 x N_t = 4096 (1 mm / 800 fs, fused
+silica, 5 uJ 40 fs w0 = 100 um with 1 % seeded amplitude noise), synthetic
+inputs built by the config front end.  A "step" is one z-step of the
+reference scheme: RK4 of the nonlinear RHS over every radial line (4 RHS
+evaluations, 16 line FFTs, plasma and Raman scans), the max-norm reductions of
+the step controller, QDHT + linear step, QDHT back.  Throughput is measured
+in fixed-dz mode (the same dz sequence for every precision) so that step-count
+differences of the adaptive controller do not enter the rate.
+
+Prints one JSON line: value = z-steps/s of the mixed-precision march
+(split-BF16 MFMA QDHT, FP32 FFT, FP64 plasma scan / propagator phase / norms),
+with the plain-FP32 rate, both fields' relative L2 error against an FP64 GPU
+run of the same steps, the roofline of the dominant kernel, and the oracle
+(the CPU restatement of the reference path) timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RK z-steps/s at N_r=1024×N_t=4096, mixed vs FP32 GPU at ≤1e-4 rel field err"
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"fp32": 157.3, "mixed": 2500.0, "fp64": 78.6}
+
+
+def run_precision(gsa, cfg, prec, steps, warmup, dz):
+    p = gsa.Plan(cfg, precision=prec)
+    for _ in range(warmup):
+        p.fixed_step(dz)
+    ktot = {"nl_rk4": 0.0, "qdht_lstep": 0.0, "qdht_back": 0.0, "reduce": 0.0}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        p.fixed_step(dz)
+        for k, v in p.timings().items():
+            ktot[k] += v
+    wall = time.perf_counter() - t0
+    field = p.get()
+    R, T = p.R, p.T
+    p.close()
+    return {"wall_s": wall, "steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
+            "kernel_ms": {k: v / steps for k, v in ktot.items()}, "field": field, "R": R, "T": T}
+
+
+def cpu_baseline(cfg, dz, nsteps, threads):
+    import oracle
+    sim = oracle.Sim(cfg, nthreads=threads, precision="fp64")
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        sim.fixed_step(dz)
+    dt = time.perf_counter() - t0
+    sim.close()
+    return {"value": nsteps / dt, "unit": "z-steps/s", "cores": threads, "kind": "port",
+            "sample": "%d fixed-dz z-step(s) of the FP64 oracle on the same 1024x4096 workload, %d threads, %.1f s"
+                      % (nsteps, threads, dt)}
+
+
+def roofline(kind, ms, R, T, prec):
+    if kind == "qdht":
+        flops = 4.0 * R * R * T                 # one transform: real R x R times 2T real columns
+        ach = flops / (ms * 1e-3) / 1e12
+        peak = PEAK_TFLOPS[prec]
+        return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                "traffic": None, "kernel": "k_qdht (%s), %.3f GFLOP algorithmic per launch" % (prec, flops / 1e9)}
+    nbytes = 16.0 * R * T                       # read FIELD + write BIGBUFFER1 (FP32 complex)
+    ach = nbytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": None, "kernel": "k_nl_rk4, %.1f MiB algorithmic per launch" % (nbytes / 2**20)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the z-march of one run is not sharded); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import gsolver_amd as gsa
+    from gsolver_amd import workloads
+
+    work = tempfile.mkdtemp(prefix="gs_bench_")
+    cfg = workloads.write_config(work, "c1")
+    probe = gsa.Plan(cfg, precision="fp32")
+    dz = probe.scalar("ZSTEP")
+    probe.close()
+
+    res = {}
+    for prec in ("mixed", "fp32") + (() if args.no_fp64 else ("fp64",)):
+        res[prec] = run_precision(gsa, cfg, prec, args.steps, args.warmup, dz)
+        print("[bench] %s: %.1f z-steps/s, kernels %s" % (prec, res[prec]["steps_per_s"],
+              {k: round(v, 4) for k, v in res[prec]["kernel_ms"].items()}), file=sys.stderr, flush=True)
+
+    err = {}
+    if "fp64" in res:
+        ref = res["fp64"]["field"]
+        for prec in ("mixed", "fp32"):
+            err[prec] = float(np.linalg.norm(res[prec]["field"] - ref) / np.linalg.norm(ref))
+
+    m = res["mixed"]
+    km = m["kernel_ms"]
+    R, T = m["R"], m["T"]
+    qdht_ms = 0.5 * (km["qdht_lstep"] + km["qdht_back"])
+    dominant = "nl" if km["nl_rk4"] >= km["qdht_lstep"] + km["qdht_back"] else "qdht"
+    roof = roofline("qdht" if dominant == "qdht" else "nl", qdht_ms if dominant == "qdht" else km["nl_rk4"], R, T, "mixed")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline(cfg, dz, args.cpu_steps, threads)
+
+    out = {
+        "metric": METRIC,
+        "value": m["steps_per_s"],
+        "unit": "z-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": m["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "mixed: bf16x3-mfma/f32-acc QDHT, f32 FFT, f64 plasma scan",
+        "data": "synthetic (config-built analytic pulse, seeded 1% amplitude noise)",
+        "config": {"workload": "N_r=1024 x N_t=4096, R_MAX 1 mm, 800 fs window, fused silica, 5 uJ 40 fs w0=100 um",
+                   "grid": [R, T], "dz_m": dz, "mode": "fixed-dz", "precision": "mixed"},
+        "fp32_value": res["fp32"]["steps_per_s"],
+        "mixed_over_fp32": m["steps_per_s"] / res["fp32"]["steps_per_s"],
+        "field_rel_l2_vs_fp64": err or None,
+        "kernel_ms_per_step": {p: res[p]["kernel_ms"] for p in res},
+        "roofline": roof,
+        "roofline_qdht": roofline("qdht", qdht_ms, R, T, "mixed"),
+        "roofline_qdht_fp32": roofline("qdht", 0.5 * (res["fp32"]["kernel_ms"]["qdht_lstep"] +
+                                                      res["fp32"]["kernel_ms"]["qdht_back"]), R, T, "fp32"),
+        "roofline_nl": roofline("nl", km["nl_rk4"], R, T, "mixed"),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

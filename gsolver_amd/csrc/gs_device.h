@@ -1,0 +1,85 @@
+// gs_device.h -- device-side data of a plan and the launchers of the gfx950
+// kernels in gs_device.hip.  Host code (gs_plan.cpp) fills these structs and
+// calls the launchers; nothing here is visible through the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gs {
+
+enum Prec { kFP64 = 0, kFP32 = 1, kMIXED = 2 };
+
+// status bits written by kernels (atomicOr into DevStatus::flags)
+enum : int {
+    kFlagBlast = 1,      // ionization table index beyond 8190 (ionization.cpp:114-118)
+    kFlagNaN = 2,        // NaN intensity (ionization.cpp:104-105)
+    kFlagScanSlow = 4    // plasma scan needed the chunk-sequential fallback
+};
+
+struct DevStatus {
+    double maxA2, maxNL2;
+    int flags;
+    int fallbacks;
+};
+
+// Scalars and table pointers the NL kernels read.  Table pointers refer to
+// device arrays in the plan's field precision unless noted.
+struct NLArgs {
+    int nr;                     // number of real radial lines (<= padded rows)
+    double inv_nt;              // 1/N_T
+    // plasma (solver units)
+    double dt;                  // TSTEP * 1e20
+    double neutral, inv_neutral, inv_recomb, U, pond, sigma, rho_init;
+    const double* ionrate;      // 8192 FP64 ln W table
+    // Raman / Kerr
+    int use_kerr, use_raman;
+    double one_m_fr, frM;
+    double onemzT_re, onemzT_im;   // 1 - z^T
+    const void* zpow;           // z^k, k = 0..T (C2<R>)
+    const void* pf;             // PLASMA_FUNC (C2<R>)
+    const void* kerr;           // KERR_PROFILE (R)
+    const void* tw;             // e^{-2 pi i k/T}, k < T (C2<R>)
+    DevStatus* status;
+};
+
+struct LstepArgs {
+    int nr;                     // rows >= nr are padding: written as zero
+    double dz;
+    const double* kt2;          // per row (padded)
+    const double* k1;           // Re k(omega)
+    const double* k2;           // Im k(omega)
+    const double* ho;           // HO_DISPERSION
+    const double* lim;          // MAX_KT2 * Re(k*k)
+    const unsigned char* wok;   // OMEGA_MIN <= omega <= OMEGA_MAX
+    void* b2;                   // BIGBUFFER2 (C2<R>), updated in place
+    void* fk;                   // FIELD in k-space (C2<R>), written
+};
+
+// ---- launchers (all asynchronous on `st`) ----
+// RK4 NL step over nr lines of field -> b1; per-line maxima into linemax[2*r]
+int launch_nl_rk4(Prec p, int nt, const void* field, void* b1, double dz, const NLArgs& a, double* linemax, hipStream_t st);
+// one RHS evaluation per line (calculate_NLresponse)
+int launch_nl_response(Prec p, int nt, const void* in, void* out, int nlines, const NLArgs& a, hipStream_t st);
+// batched line FFT, optional scale and spectral filter (double per omega, may be null)
+int launch_fft_lines(Prec p, int nt, const void* in, void* out, int nlines, int sign, double scale,
+                     const double* wfilter, const void* tw, hipStream_t st);
+// per-line diagnostics of a spectral buffer: flux, maxI (with filter), rhomax (unfiltered);
+// optional E(t) rows (et, scaled 1/T, unfiltered) and rho(t) rows (rho, double)
+int launch_line_diag(Prec p, int nt, const void* field, int nlines, const double* wfilter, double tstep,
+                     const NLArgs& a, double* flux, double* maxI, double* rhomax, void* et, double* rho, hipStream_t st);
+// QDHT GEMM: out = A * in, A = diag(m1) C diag(1/m1) padded to rp x rp.
+// amat: FP64/FP32 transposed matrix, or the hi/lo bf16 pair for MIXED.
+int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* amat_lo, const void* in, void* out,
+                const LstepArgs* lstep, hipStream_t st);
+// element-wise linear step from a k-space increment (calculate_Lstep)
+int launch_lstep(Prec p, int rp, int nt, const void* b1k, const LstepArgs& la, hipStream_t st);
+// max of the per-line maxima -> status
+int launch_reduce_max(int nr, const double* linemax, DevStatus* st_out, hipStream_t st);
+// conversions between the host FP64 layout and the device field precision
+int launch_convert(Prec p, const double* src64, void* dst, size_t n_complex, int to_device, hipStream_t st);
+// average spectrum (dump_types.cpp:429-457): out[w] = XMAX^2/2 sum_r |F_rw|^2 x_r (x_r - x_{r-1})
+int launch_avg_spectrum(Prec p, int nr, int nt, const void* field, const double* wts, double* out, hipStream_t st);
+
+}  // namespace gs

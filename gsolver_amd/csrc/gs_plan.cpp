@@ -1,0 +1,870 @@
+// gs_plan.cpp -- the plan object and the C-ABI of include/gsolver_amd.h.
+//
+// The host side replays the reference's control flow: process_input
+// (set-up), the propagator() accept/reject controller (propagator.cu:125-266),
+// the main loop with dump planes (solver_main.cpp:202-225) and the .udf
+// writers (dump_types.cpp:25-72); every per-step array operation runs on the
+// GPU through gs_device.hip.  There is no CPU fallback: without a device the
+// plan cannot be created.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/gsolver_amd.h"
+#include "gs_device.h"
+#include "gs_host.h"
+
+using gs::cd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& m)
+{
+    g_err = m;
+    return code;
+}
+
+#define HCK(x)                                                                                    \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) throw gs::Error(GS_E_HIP, std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
+    } while (0)
+#define LCK(x)                                                                                    \
+    do {                                                                                          \
+        int e_ = (x);                                                                             \
+        if (e_ != 0) throw gs::Error(GS_E_HIP, "kernel launch failed (" + std::to_string(e_) + ") at " #x); \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    void alloc(size_t bytes)
+    {
+        if (p) HCK(hipFree(p));
+        p = nullptr;
+        n = bytes;
+        if (bytes) HCK(hipMalloc(&p, bytes));
+    }
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+void upload(DevBuf& b, const void* src, size_t bytes)
+{
+    b.alloc(bytes);
+    HCK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+}
+
+uint16_t bf16_rn_host(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+float bf16_to_float(uint16_t h)
+{
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+}  // namespace
+
+struct gs_plan {
+    gs::Setup s;
+    gs::Prec prec = gs::kFP32;
+    int rp = 0;              // padded rows
+    size_t relem = 0;        // bytes per complex element in the field precision
+    hipStream_t st = nullptr;
+    hipEvent_t ev[5] = {};
+
+    DevBuf field, b1, b2, fk, tmp;       // [rp][T] in field precision
+    DevBuf amat, amat_lo, at64;          // QDHT operators
+    DevBuf tw, tw64, zpow, pf, kerr, ionrate, kt2, k1, k2, ho, lim, wok, avgw;
+    DevBuf linemax, status, diag, et, rhot, scratch64a, scratch64b;
+    std::vector<DevBuf> wfilt;
+    gs::NLArgs nl{};
+    gs::LstepArgs ls{};
+
+    double ZSTEP = 0, CURRENT_Z = 0;
+    long STEP_N = 0, n_reject = 0;
+    int n_Z = 0;
+    double timings[4] = {0, 0, 0, 0};
+
+    void* buf(int id)
+    {
+        switch (id) {
+        case GS_FIELD: return field.p;
+        case GS_B1: return b1.p;
+        case GS_B2: return b2.p;
+        case GS_FK: return fk.p;
+        }
+        throw gs::Error(GS_E_ARG, "unknown buffer id");
+    }
+    size_t field_bytes() const { return (size_t)rp * s.NT * relem; }
+};
+
+namespace {
+
+void sync(gs_plan* p) { HCK(hipStreamSynchronize(p->st)); }
+
+void check_status(gs_plan* p, gs::DevStatus* out)
+{
+    gs::DevStatus h;
+    HCK(hipMemcpyAsync(&h, p->status.p, sizeof h, hipMemcpyDeviceToHost, p->st));
+    sync(p);
+    if (h.flags & gs::kFlagNaN) {
+        HCK(hipMemsetAsync(p->status.p, 0, sizeof h, p->st));
+        throw gs::Error(GS_E_PHYSICS, "float_type photoionization_function(...): Intensity is nan!");
+    }
+    if (h.flags & gs::kFlagBlast) {
+        HCK(hipMemsetAsync(p->status.p, 0, sizeof h, p->st));
+        throw gs::Error(GS_E_PHYSICS, "Attention!!!! Intensity is too big! Keep away, blast is possible.");
+    }
+    if (out) *out = h;
+}
+
+// batched FP64 line FFT on device scratch (init-time transforms)
+void fft64_lines(gs_plan* p, const double* host_in, double* host_out, int nlines, int sign, double scale)
+{
+    const size_t bytes = (size_t)nlines * p->s.NT * 16;
+    if (p->scratch64a.n < bytes) p->scratch64a.alloc(bytes);
+    if (p->scratch64b.n < bytes) p->scratch64b.alloc(bytes);
+    HCK(hipMemcpyAsync(p->scratch64a.p, host_in, bytes, hipMemcpyHostToDevice, p->st));
+    LCK(gs::launch_fft_lines(gs::kFP64, p->s.NT, p->scratch64a.p, p->scratch64b.p, nlines, sign, scale, nullptr, p->tw64.p, p->st));
+    HCK(hipMemcpyAsync(host_out, p->scratch64b.p, bytes, hipMemcpyDeviceToHost, p->st));
+    sync(p);
+}
+
+void upload_field64(gs_plan* p, DevBuf& dst, const double* host)   // host [R][T] cplx -> device [rp][T]
+{
+    const size_t n = (size_t)p->s.NR * p->s.NT;
+    if (p->scratch64a.n < n * 16) p->scratch64a.alloc((size_t)p->rp * p->s.NT * 16);
+    HCK(hipMemcpyAsync(p->scratch64a.p, host, n * 16, hipMemcpyHostToDevice, p->st));
+    HCK(hipMemsetAsync(dst.p, 0, p->field_bytes(), p->st));
+    LCK(gs::launch_convert(p->prec, p->scratch64a.as<double>(), dst.p, n, 1, p->st));
+    sync(p);
+}
+
+void download_field64(gs_plan* p, const void* src, double* host, size_t nlines)
+{
+    const size_t n = nlines * p->s.NT;
+    if (p->scratch64a.n < n * 16) p->scratch64a.alloc((size_t)p->rp * p->s.NT * 16);
+    LCK(gs::launch_convert(p->prec, p->scratch64a.as<double>(), const_cast<void*>(src), n, 0, p->st));
+    HCK(hipMemcpyAsync(host, p->scratch64a.p, n * 16, hipMemcpyDeviceToHost, p->st));
+    sync(p);
+}
+
+void build_plan(gs_plan* p)
+{
+    const gs::Setup& s = p->s;
+    const int nt = s.NT, nr = s.NR;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        throw gs::Error(GS_E_NODEVICE, "no GPU visible: gsolver_amd has no CPU fallback");
+    if (nt < 256 || nt > 4096 || (nt & (nt - 1)))
+        throw gs::Error(GS_E_ARG, "N_T must be a power of two between 256 and 4096 on the GPU path");
+    p->rp = (nr + 127) / 128 * 128;
+    const int rp = p->rp;
+    const bool dbl = p->prec == gs::kFP64;
+    p->relem = dbl ? 16 : 8;
+    HCK(hipStreamCreate(&p->st));
+    for (auto& e : p->ev) HCK(hipEventCreate(&e));
+
+    // ---- operators: A = diag(m1) C diag(1/m1), padded; stored transposed (At[k][m] = A[m][k])
+    std::vector<double> A((size_t)rp * rp, 0.0);
+    for (int i = 0; i < nr; ++i)
+        for (int j = 0; j < nr; ++j) A[(size_t)i * rp + j] = s.qm1[i] * s.qC[(size_t)i * nr + j] / s.qm1[j];
+    std::vector<double> At((size_t)rp * rp);
+    for (int i = 0; i < rp; ++i)
+        for (int j = 0; j < rp; ++j) At[(size_t)j * rp + i] = A[(size_t)i * rp + j];
+    upload(p->at64, At.data(), At.size() * 8);
+    if (p->prec == gs::kFP64) {
+        p->amat.alloc(0);
+    } else if (p->prec == gs::kFP32) {
+        std::vector<float> f(At.begin(), At.end());
+        upload(p->amat, f.data(), f.size() * 4);
+    } else {
+        // split-BF16: hi = bf16(a), lo = bf16(a - hi), row-major A[m][k]
+        std::vector<uint16_t> hi(A.size()), lo(A.size());
+        for (size_t i = 0; i < A.size(); ++i) {
+            const float a = (float)A[i];
+            hi[i] = bf16_rn_host(a);
+            lo[i] = bf16_rn_host(a - bf16_to_float(hi[i]));
+        }
+        upload(p->amat, hi.data(), hi.size() * 2);
+        upload(p->amat_lo, lo.data(), lo.size() * 2);
+    }
+
+    // ---- tables
+    std::vector<double> tw64(2 * nt);
+    for (int k = 0; k < nt; ++k) {
+        // e^{-2 pi i k / T}, evaluated by octant symmetry for accuracy
+        const double ang = 2.0 * M_PI * k / nt;
+        tw64[2 * k] = std::cos(ang);
+        tw64[2 * k + 1] = -std::sin(ang);
+        if (4 * k == nt) { tw64[2 * k] = 0; tw64[2 * k + 1] = -1; }
+        if (2 * k == nt) { tw64[2 * k] = -1; tw64[2 * k + 1] = 0; }
+        if (4 * k == 3 * nt) { tw64[2 * k] = 0; tw64[2 * k + 1] = 1; }
+    }
+    upload(p->tw64, tw64.data(), tw64.size() * 8);
+    std::vector<double> zp(2 * (nt + 1)), pf(2 * nt), kerr(s.kerr);
+    for (int k = 0; k <= nt; ++k) {
+        const cd z = std::pow(s.raman_z, (double)k);
+        zp[2 * k] = z.real();
+        zp[2 * k + 1] = z.imag();
+    }
+    for (int k = 0; k < nt; ++k) {
+        pf[2 * k] = s.plasma_func[k].real();
+        pf[2 * k + 1] = s.plasma_func[k].imag();
+    }
+    if (dbl) {
+        upload(p->tw, tw64.data(), tw64.size() * 8);
+        upload(p->zpow, zp.data(), zp.size() * 8);
+        upload(p->pf, pf.data(), pf.size() * 8);
+        upload(p->kerr, kerr.data(), kerr.size() * 8);
+    } else {
+        std::vector<float> a(tw64.begin(), tw64.end()), b(zp.begin(), zp.end()), c(pf.begin(), pf.end()), d(kerr.begin(), kerr.end());
+        upload(p->tw, a.data(), a.size() * 4);
+        upload(p->zpow, b.data(), b.size() * 4);
+        upload(p->pf, c.data(), c.size() * 4);
+        upload(p->kerr, d.data(), d.size() * 4);
+    }
+    upload(p->ionrate, s.ionrate_ln.data(), s.ionrate_ln.size() * 8);
+    std::vector<double> kt2(rp, 0.0), k1(nt), k2(nt), lim(nt);
+    std::vector<unsigned char> wok(nt);
+    for (int i = 0; i < nr; ++i) {
+        const double kt = 2 * M_PI * s.qx[i] * s.qV / s.XMAX;
+        kt2[i] = kt * kt;
+    }
+    for (int w = 0; w < nt; ++w) {
+        k1[w] = s.wavenum[w].real();
+        k2[w] = s.wavenum[w].imag();
+        lim[w] = 0.25 * (s.wavenum[w] * s.wavenum[w]).real();
+        wok[w] = !(s.omega[w] < s.OMEGA_MIN || s.omega[w] > s.OMEGA_MAX);
+    }
+    upload(p->kt2, kt2.data(), kt2.size() * 8);
+    upload(p->k1, k1.data(), k1.size() * 8);
+    upload(p->k2, k2.data(), k2.size() * 8);
+    upload(p->ho, s.ho.data(), s.ho.size() * 8);
+    upload(p->lim, lim.data(), lim.size() * 8);
+    upload(p->wok, wok.data(), wok.size());
+    {
+        // average_spectrum weights: x0 (x1 - x0), then x_r (x_r - x_{r-1})
+        std::vector<double> wts(nr);
+        for (int r = 0; r < nr; ++r) {
+            const double x1 = r == 0 ? s.qx[0] : s.qx[r - 1];
+            const double x2 = r == 0 ? s.qx[1] : s.qx[r];
+            wts[r] = (r == 0 ? x1 : x2) * (x2 - x1);
+        }
+        upload(p->avgw, wts.data(), wts.size() * 8);
+    }
+    for (const auto& d : s.dumps) {
+        p->wfilt.emplace_back();
+        if (!d.wfilter.empty()) upload(p->wfilt.back(), d.wfilter.data(), d.wfilter.size() * 8);
+    }
+
+    // ---- state buffers
+    for (DevBuf* b : {&p->field, &p->b1, &p->b2, &p->fk, &p->tmp}) {
+        b->alloc(p->field_bytes());
+        HCK(hipMemset(b->p, 0, p->field_bytes()));
+    }
+    p->linemax.alloc((size_t)rp * 2 * 8);
+    p->status.alloc(sizeof(gs::DevStatus));
+    HCK(hipMemset(p->status.p, 0, sizeof(gs::DevStatus)));
+    p->diag.alloc((size_t)rp * 3 * 8);
+
+    // ---- kernel argument blocks
+    gs::NLArgs& a = p->nl;
+    a.nr = nr;
+    a.inv_nt = 1.0 / nt;
+    a.dt = s.TSTEP * 1e20;
+    a.neutral = s.neutral_density;
+    a.inv_neutral = 1.0 / s.neutral_density;
+    a.inv_recomb = 1.0 / s.recomb_tau;
+    a.U = s.ion_potential;
+    a.pond = s.ponderomotive;
+    a.sigma = s.avalanche;
+    a.rho_init = s.rho_init;
+    a.ionrate = p->ionrate.as<double>();
+    a.use_kerr = s.n2 != 0;
+    a.use_raman = s.raman_fraction > 0.001;
+    a.one_m_fr = 1 - s.raman_fraction;
+    a.frM = s.raman_fraction * s.raman_M;
+    const cd omzT = 1.0 - std::pow(s.raman_z, (double)nt);
+    a.onemzT_re = omzT.real();
+    a.onemzT_im = omzT.imag();
+    a.zpow = p->zpow.p;
+    a.pf = p->pf.p;
+    a.kerr = p->kerr.p;
+    a.tw = p->tw.p;
+    a.status = p->status.as<gs::DevStatus>();
+    gs::LstepArgs& l = p->ls;
+    l.nr = nr;
+    l.kt2 = p->kt2.as<double>();
+    l.k1 = p->k1.as<double>();
+    l.k2 = p->k2.as<double>();
+    l.ho = p->ho.as<double>();
+    l.lim = p->lim.as<double>();
+    l.wok = p->wok.as<unsigned char>();
+    l.b2 = p->b2.p;
+    l.fk = p->fk.p;
+
+    // ---- initial field: FFT of E(r,t), spectral phase, /FIELD_DENOM (pulse_gg.cpp:100-121, process_input.cpp:58)
+    const size_t n = (size_t)nr * nt;
+    std::vector<double> f64(2 * n), sp(2 * n);
+    std::memcpy(f64.data(), s.field_t.data(), n * 16);
+    fft64_lines(p, f64.data(), sp.data(), nr, -1, 1.0);
+    for (int r = 0; r < nr; ++r)
+        for (int w = 0; w < nt; ++w) {
+            const size_t o = (size_t)r * nt + w;
+            const cd v = cd(sp[2 * o], sp[2 * o + 1]) * gs::init_spectral_factor(s, r, w) / 1e8;
+            sp[2 * o] = v.real();
+            sp[2 * o + 1] = v.imag();
+        }
+    // init_zstep_kerr (process_input.cpp:734-753): max |IFFT(FIELD)/T|^2
+    fft64_lines(p, sp.data(), f64.data(), nr, +1, 1.0 / nt);
+    double maxI = 0;
+    for (size_t i = 0; i < n; ++i) maxI = std::max(maxI, f64[2 * i] * f64[2 * i] + f64[2 * i + 1] * f64[2 * i + 1]);
+    p->ZSTEP = s.ZNET[1] - s.ZNET[0];
+    const double kNL = s.OMEGA0 / 299792458.0 * s.n2 * maxI;
+    if (p->ZSTEP * kNL > s.MAX_TOLERANCE) p->ZSTEP = s.MAX_TOLERANCE / kNL;
+    p->CURRENT_Z = s.CURRENT_Z0;
+    upload_field64(p, p->field, sp.data());
+    // BIGBUFFER2 = QDHT(FIELD) in FP64
+    {
+        const size_t bytes = (size_t)rp * nt * 16;
+        if (p->scratch64a.n < bytes) p->scratch64a.alloc(bytes);
+        if (p->scratch64b.n < bytes) p->scratch64b.alloc(bytes);
+        HCK(hipMemsetAsync(p->scratch64a.p, 0, bytes, p->st));
+        HCK(hipMemcpyAsync(p->scratch64a.p, sp.data(), n * 16, hipMemcpyHostToDevice, p->st));
+        LCK(gs::launch_qdht(gs::kFP64, rp, nt, p->at64.p, nullptr, p->scratch64a.p, p->scratch64b.p, nullptr, p->st));
+        LCK(gs::launch_convert(p->prec, p->scratch64b.as<double>(), p->b2.p, (size_t)rp * nt, 1, p->st));
+        sync(p);
+    }
+    p->s.field_t.clear();
+    p->s.field_t.shrink_to_fit();
+}
+
+// ---- one NL attempt + controller pieces
+void nl_step(gs_plan* p, double dz, double* maxA2, double* maxNL2)
+{
+    HCK(hipEventRecord(p->ev[0], p->st));
+    LCK(gs::launch_nl_rk4(p->prec, p->s.NT, p->field.p, p->b1.p, dz, p->nl, p->linemax.as<double>(), p->st));
+    HCK(hipEventRecord(p->ev[1], p->st));
+    LCK(gs::launch_reduce_max(p->s.NR, p->linemax.as<double>(), p->status.as<gs::DevStatus>(), p->st));
+    HCK(hipEventRecord(p->ev[4], p->st));
+    gs::DevStatus h;
+    check_status(p, &h);
+    *maxA2 = h.maxA2;
+    *maxNL2 = h.maxNL2;
+}
+
+void linear_half(gs_plan* p, double dz)
+{
+    p->ls.dz = dz;
+    HCK(hipEventRecord(p->ev[2], p->st));
+    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, p->prec == gs::kFP64 ? p->at64.p : p->amat.p, p->amat_lo.p, p->b1.p,
+                        nullptr, &p->ls, p->st));
+    HCK(hipEventRecord(p->ev[3], p->st));
+    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, p->prec == gs::kFP64 ? p->at64.p : p->amat.p, p->amat_lo.p, p->fk.p,
+                        p->field.p, nullptr, p->st));
+}
+
+void finish_timings(gs_plan* p, hipEvent_t end)
+{
+    float a = 0, b = 0, c = 0, d = 0;
+    sync(p);
+    (void)hipEventElapsedTime(&a, p->ev[0], p->ev[1]);
+    (void)hipEventElapsedTime(&d, p->ev[1], p->ev[4]);
+    (void)hipEventElapsedTime(&b, p->ev[2], p->ev[3]);
+    (void)hipEventElapsedTime(&c, p->ev[3], end);
+    p->timings[0] = a;
+    p->timings[1] = b;
+    p->timings[2] = c;
+    p->timings[3] = d;
+}
+
+// propagator(), propagator.cu:125-266 (apertures unsupported)
+void propagator(gs_plan* p)
+{
+    const auto& Z = p->s.ZNET;
+    const double tol = p->s.MAX_TOLERANCE;
+    for (;;) {
+        double maxA2, maxNL2;
+        nl_step(p, p->ZSTEP, &maxA2, &maxNL2);
+        bool repeat = false;
+        if (p->ZSTEP * p->ZSTEP * maxNL2 > tol * tol * maxA2) { repeat = true; p->STEP_N++; p->n_reject++; }
+        p->ZSTEP = 0.5 * tol * std::sqrt(maxA2 / maxNL2);
+        const int nz = std::max(p->n_Z, 1);
+        if (p->ZSTEP < 1e-10 * (Z[nz] - Z[nz - 1]) || !(p->ZSTEP > 0))
+            throw gs::Error(GS_E_PHYSICS, "Minimum Z step size reached, collapse is possible!");
+        p->ZSTEP = std::min(p->ZSTEP, Z[nz] - p->CURRENT_Z);
+        if (!repeat) {
+            linear_half(p, p->ZSTEP);
+            hipEvent_t end;
+            HCK(hipEventCreate(&end));
+            HCK(hipEventRecord(end, p->st));
+            finish_timings(p, end);
+            (void)hipEventDestroy(end);
+            return;
+        }
+    }
+}
+
+// ---- .udf output (dump_types.cpp:25-72)
+struct Udf {
+    std::FILE* f = nullptr;
+    int cplx = 0;
+    Udf() = default;
+    Udf(const Udf&) = delete;
+    Udf& operator=(const Udf&) = delete;
+    Udf(Udf&& o) noexcept : f(o.f), cplx(o.cplx) { o.f = nullptr; }
+    ~Udf()
+    {
+        if (f) std::fclose(f);
+    }
+    void open(const std::string& path, int iscomplex, int tN, int xN, int yN, int zN)
+    {
+        f = std::fopen(path.c_str(), "w+b");
+        if (!f) throw gs::Error(GS_E_CONFIG, "void dump_type_empty_class :: init_dumpfile : Unable to open file!");
+        cplx = iscomplex;
+        const char tag[] = "UDF1.0";
+        const int h[6] = {8, iscomplex, tN, xN, yN, zN};
+        std::fwrite(tag, 1, sizeof tag, f);
+        std::fwrite(h, sizeof(int), 6, f);
+        std::fflush(f);
+    }
+    void write(long long elem_ofs, const double* data, size_t nelem)
+    {
+        const long long off = 31 + elem_ofs * (cplx ? 16 : 8);
+        fseeko(f, off, SEEK_SET);
+        std::fwrite(data, cplx ? 16 : 8, nelem, f);
+        std::fflush(f);
+    }
+};
+
+struct DumpSet {
+    std::vector<Udf> files;
+};
+
+void open_dumps(gs_plan* p, const std::string& outdir, DumpSet* ds)
+{
+    const auto& s = p->s;
+    ds->files.resize(s.dumps.size());
+    for (size_t i = 0; i < s.dumps.size(); ++i) {
+        const auto& d = s.dumps[i];
+        const std::string path = outdir + "/" + d.filename;
+        if (d.type == "full") ds->files[i].open(path, 1, s.NT, s.NR, 1, s.NZ / d.nz_denom);
+        else if (d.type == "maxI" || d.type == "flux" || d.type == "plasma_max") ds->files[i].open(path, 0, 1, s.NR, 1, s.NZ);
+        else if (d.type == "field_axis") ds->files[i].open(path, 1, s.NT, 1, 1, s.NZ);
+        else ds->files[i].open(path, 0, s.NT, 1, 1, s.NZ);   // average_spectrum, plasma_axis
+    }
+}
+
+void do_dumps(gs_plan* p, DumpSet* ds)
+{
+    const auto& s = p->s;
+    const int nr = s.NR, nt = s.NT, nz = p->n_Z;
+    double* dflux = p->diag.as<double>();
+    double* dmax = dflux + p->rp;
+    double* drho = dmax + p->rp;
+    bool have_plain = false;
+    std::vector<double> flux(nr), maxI(nr), rho(nr);
+    for (size_t i = 0; i < s.dumps.size(); ++i) {
+        const auto& d = s.dumps[i];
+        Udf& u = ds->files[i];
+        bool filtered = false;
+        for (double v : d.wfilter) filtered |= v != 1.0;
+        if (d.type == "flux" || d.type == "maxI" || d.type == "plasma_max") {
+            if (filtered) {
+                LCK(gs::launch_line_diag(p->prec, nt, p->field.p, nr, p->wfilt[i].as<double>(), s.TSTEP, p->nl, dflux, dmax, drho,
+                                         nullptr, nullptr, p->st));
+                std::vector<double> o(nr);
+                HCK(hipMemcpyAsync(o.data(), d.type == "flux" ? dflux : dmax, nr * 8, hipMemcpyDeviceToHost, p->st));
+                check_status(p, nullptr);
+                u.write((long long)nr * nz, o.data(), nr);
+                have_plain = false;
+                continue;
+            }
+            if (!have_plain) {
+                LCK(gs::launch_line_diag(p->prec, nt, p->field.p, nr, nullptr, s.TSTEP, p->nl, dflux, dmax, drho, nullptr,
+                                         nullptr, p->st));
+                HCK(hipMemcpyAsync(flux.data(), dflux, nr * 8, hipMemcpyDeviceToHost, p->st));
+                HCK(hipMemcpyAsync(maxI.data(), dmax, nr * 8, hipMemcpyDeviceToHost, p->st));
+                HCK(hipMemcpyAsync(rho.data(), drho, nr * 8, hipMemcpyDeviceToHost, p->st));
+                check_status(p, nullptr);
+                have_plain = true;
+            }
+            const std::vector<double>& o = d.type == "flux" ? flux : (d.type == "maxI" ? maxI : rho);
+            u.write((long long)nr * nz, o.data(), nr);
+        } else if (d.type == "field_axis" || d.type == "full") {
+            if (d.type == "full" && nz % d.nz_denom) continue;
+            const int lines = d.type == "full" ? nr : 1;
+            if (p->et.n < (size_t)lines * nt * p->relem) p->et.alloc((size_t)p->rp * nt * p->relem);
+            LCK(gs::launch_fft_lines(p->prec, nt, p->field.p, p->et.p, lines, +1, 1.0 / nt, nullptr, p->tw.p, p->st));
+            std::vector<double> h((size_t)lines * nt * 2);
+            download_field64(p, p->et.p, h.data(), lines);
+            if (d.type == "full") u.write((long long)nt * ((long long)nr * (nz / d.nz_denom)), h.data(), (size_t)lines * nt);
+            else u.write((long long)nt * nz, h.data(), nt);
+        } else if (d.type == "average_spectrum") {
+            if (p->rhot.n < (size_t)nt * 8) p->rhot.alloc((size_t)std::max(nt, p->rp) * nt * 8);
+            LCK(gs::launch_avg_spectrum(p->prec, nr, nt, p->field.p, p->avgw.as<double>(), p->rhot.as<double>(), p->st));
+            std::vector<double> h(nt);
+            HCK(hipMemcpyAsync(h.data(), p->rhot.p, nt * 8, hipMemcpyDeviceToHost, p->st));
+            sync(p);
+            for (double& v : h) v *= s.XMAX * s.XMAX / 2.0;
+            u.write((long long)nt * nz, h.data(), nt);
+        } else if (d.type == "plasma_axis") {
+            if (p->rhot.n < (size_t)nt * 8) p->rhot.alloc((size_t)std::max(nt, p->rp) * nt * 8);
+            LCK(gs::launch_line_diag(p->prec, nt, p->field.p, 1, nullptr, s.TSTEP, p->nl, nullptr, nullptr, nullptr, nullptr,
+                                     p->rhot.as<double>(), p->st));
+            std::vector<double> h(nt);
+            HCK(hipMemcpyAsync(h.data(), p->rhot.p, nt * 8, hipMemcpyDeviceToHost, p->st));
+            check_status(p, nullptr);
+            u.write((long long)nt * nz, h.data(), nt);
+        }
+    }
+}
+
+template <class F>
+int guarded(F&& f)
+{
+    try {
+        return f();
+    } catch (const gs::Error& e) {
+        return fail(e.code, e.what());
+    } catch (const std::exception& e) {
+        return fail(GS_E_ARG, e.what());
+    }
+}
+
+}  // namespace
+
+// ====================================================================== C-ABI
+extern "C" {
+
+const char* gs_last_error(void) { return g_err.c_str(); }
+
+int gs_plan_create(const char* cfg_path, int nover, const char* const* over, int precision, const char* workdir, gs_plan** out)
+{
+    return guarded([&]() {
+        if (!cfg_path || !out) return fail(GS_E_ARG, "null argument");
+        if (precision < 0 || precision > 2) return fail(GS_E_ARG, "precision must be GS_FP64, GS_FP32 or GS_MIXED");
+        std::vector<std::string> ov;
+        for (int i = 0; i < nover; ++i) ov.emplace_back(over[i]);
+        std::unique_ptr<gs_plan> p(new gs_plan);
+        p->prec = (gs::Prec)precision;
+        gs::Config cfg(cfg_path, ov);
+        p->s = gs::build_setup(cfg, workdir ? workdir : "");
+        build_plan(p.get());
+        *out = p.release();
+        return 0;
+    });
+}
+
+void gs_plan_destroy(gs_plan* p)
+{
+    if (!p) return;
+    if (p->st) (void)hipStreamSynchronize(p->st);
+    for (auto& e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+    hipStream_t st = p->st;
+    delete p;
+    if (st) (void)hipStreamDestroy(st);
+}
+
+int gs_dims(gs_plan* p, int* nr, int* nt, int* nz)
+{
+    if (!p) return fail(GS_E_ARG, "null plan");
+    if (nr) *nr = p->s.NR;
+    if (nt) *nt = p->s.NT;
+    if (nz) *nz = p->s.NZ;
+    return 0;
+}
+
+int gs_get_scalar(gs_plan* p, const char* name, double* v)
+{
+    if (!p || !name || !v) return fail(GS_E_ARG, "null argument");
+    const std::string n(name);
+    const auto& s = p->s;
+    if (n == "ZSTEP") *v = p->ZSTEP;
+    else if (n == "CURRENT_Z") *v = p->CURRENT_Z;
+    else if (n == "STEP_N") *v = (double)p->STEP_N;
+    else if (n == "n_Z") *v = p->n_Z;
+    else if (n == "n_reject") *v = (double)p->n_reject;
+    else if (n == "OMEGA0") *v = s.OMEGA0;
+    else if (n == "OMEGA_MIN") *v = s.OMEGA_MIN;
+    else if (n == "OMEGA_MAX") *v = s.OMEGA_MAX;
+    else if (n == "GROUP_VELOCITY") *v = s.GROUP_VELOCITY;
+    else if (n == "TSTEP") *v = s.TSTEP;
+    else if (n == "TMIN") *v = s.TMIN;
+    else if (n == "TMAX") *v = s.TMAX;
+    else if (n == "XMAX") *v = s.XMAX;
+    else if (n == "MAX_TOLERANCE") *v = s.MAX_TOLERANCE;
+    else if (n == "NONLIN_REFRINDEX") *v = s.n2;
+    else if (n == "AVALANCHE_CROSSSECTION") *v = s.avalanche;
+    else if (n == "PONDEROMOTIVE_COEFFICIENT") *v = s.ponderomotive;
+    else if (n == "IONIZATION_POTENTIAL") *v = s.ion_potential;
+    else if (n == "RECOMBINATION_TAU") *v = s.recomb_tau;
+    else if (n == "NEUTRAL_DENSITY") *v = s.neutral_density;
+    else if (n == "qS") *v = s.qS;
+    else if (n == "precision") *v = p->prec;
+    else if (n == "WAVENUMBER0_re") *v = s.WAVENUMBER0.real();
+    else return fail(GS_E_ARG, "unknown scalar " + n);
+    return 0;
+}
+
+int gs_set_scalar(gs_plan* p, const char* name, double v)
+{
+    if (!p || !name) return fail(GS_E_ARG, "null argument");
+    const std::string n(name);
+    if (n == "ZSTEP") p->ZSTEP = v;
+    else if (n == "CURRENT_Z") p->CURRENT_Z = v;
+    else if (n == "n_Z") p->n_Z = (int)v;
+    else if (n == "MAX_TOLERANCE") p->s.MAX_TOLERANCE = v;
+    else return fail(GS_E_ARG, "scalar " + n + " is not settable");
+    return 0;
+}
+
+static int copy_table(const gs::Setup& s, const char* name, double* out, size_t n);
+
+int gs_get_table(gs_plan* p, const char* name, double* out, size_t n)
+{
+    if (!p || !name || !out) return fail(GS_E_ARG, "null argument");
+    return copy_table(p->s, name, out, n);
+}
+
+int gs_setup_table(const char* cfg_path, int nover, const char* const* over, const char* name, double* out, size_t n)
+{
+    return guarded([&]() {
+        if (!cfg_path || !name || !out) return fail(GS_E_ARG, "null argument");
+        std::vector<std::string> ov;
+        for (int i = 0; i < nover; ++i) ov.emplace_back(over[i]);
+        gs::Config cfg(cfg_path, ov);
+        const gs::Setup s = gs::build_setup(cfg, "");
+        return copy_table(s, name, out, n);
+    });
+}
+
+static int copy_table(const gs::Setup& s, const char* name, double* out, size_t n)
+{
+    const std::string nm(name);
+    auto put = [&](const std::vector<double>& v) {
+        if (n < v.size()) return fail(GS_E_ARG, "output too small for " + nm);
+        std::memcpy(out, v.data(), v.size() * 8);
+        return 0;
+    };
+    auto putc = [&](const std::vector<cd>& v) {
+        if (n < 2 * v.size()) return fail(GS_E_ARG, "output too small for " + nm);
+        std::memcpy(out, v.data(), v.size() * 16);
+        return 0;
+    };
+    if (nm == "omega") return put(s.omega);
+    if (nm == "wavenum") return putc(s.wavenum);
+    if (nm == "ho") return put(s.ho);
+    if (nm == "kerr") return put(s.kerr);
+    if (nm == "plasma_func") return putc(s.plasma_func);
+    if (nm == "raman_function") return putc(s.raman_function);
+    if (nm == "ionrate_ln") return put(s.ionrate_ln);
+    if (nm == "qdht_x") return put(s.qx);
+    if (nm == "qdht_m1") return put(s.qm1);
+    if (nm == "qdht_c") return put(s.qC);
+    if (nm == "znet") return put(s.ZNET);
+    return fail(GS_E_ARG, "unknown table " + nm);
+}
+
+int gs_get(gs_plan* p, int buffer, double* host)
+{
+    return guarded([&]() {
+        if (!p || !host) return fail(GS_E_ARG, "null argument");
+        download_field64(p, p->buf(buffer), host, p->s.NR);
+        return 0;
+    });
+}
+
+int gs_put(gs_plan* p, int buffer, const double* host)
+{
+    return guarded([&]() {
+        if (!p || !host) return fail(GS_E_ARG, "null argument");
+        DevBuf tmp;
+        void* dst = p->buf(buffer);
+        const size_t n = (size_t)p->s.NR * p->s.NT;
+        upload(tmp, host, n * 16);
+        HCK(hipMemsetAsync(dst, 0, p->field_bytes(), p->st));
+        LCK(gs::launch_convert(p->prec, tmp.as<double>(), dst, n, 1, p->st));
+        sync(p);
+        return 0;
+    });
+}
+
+int gs_nl_step(gs_plan* p, double dz, double* maxA2, double* maxNL2)
+{
+    return guarded([&]() {
+        if (!p) return fail(GS_E_ARG, "null plan");
+        double a, b;
+        nl_step(p, dz, &a, &b);
+        if (maxA2) *maxA2 = a;
+        if (maxNL2) *maxNL2 = b;
+        return 0;
+    });
+}
+
+int gs_nl_response(gs_plan* p, const double* in, double* out, int n_lines)
+{
+    return guarded([&]() {
+        if (!p || !in || !out || n_lines < 1) return fail(GS_E_ARG, "bad argument");
+        const int nt = p->s.NT;
+        const size_t n = (size_t)n_lines * nt;
+        DevBuf d64, din, dout;
+        upload(d64, in, n * 16);
+        din.alloc(n * p->relem);
+        dout.alloc(n * p->relem);
+        LCK(gs::launch_convert(p->prec, d64.as<double>(), din.p, n, 1, p->st));
+        LCK(gs::launch_nl_response(p->prec, nt, din.p, dout.p, n_lines, p->nl, p->st));
+        LCK(gs::launch_convert(p->prec, d64.as<double>(), dout.p, n, 0, p->st));
+        HCK(hipMemcpyAsync(out, d64.p, n * 16, hipMemcpyDeviceToHost, p->st));
+        check_status(p, nullptr);
+        return 0;
+    });
+}
+
+int gs_qdht(gs_plan* p, int buffer)
+{
+    return guarded([&]() {
+        if (!p) return fail(GS_E_ARG, "null plan");
+        void* b = p->buf(buffer);
+        LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, p->prec == gs::kFP64 ? p->at64.p : p->amat.p, p->amat_lo.p, b,
+                            p->tmp.p, nullptr, p->st));
+        HCK(hipMemcpyAsync(b, p->tmp.p, p->field_bytes(), hipMemcpyDeviceToDevice, p->st));
+        sync(p);
+        return 0;
+    });
+}
+
+int gs_linear_step(gs_plan* p, double dz)
+{
+    return guarded([&]() {
+        if (!p) return fail(GS_E_ARG, "null plan");
+        p->ls.dz = dz;
+        LCK(gs::launch_lstep(p->prec, p->rp, p->s.NT, p->b1.p, p->ls, p->st));
+        sync(p);
+        return 0;
+    });
+}
+
+int gs_step(gs_plan* p)
+{
+    return guarded([&]() {
+        if (!p) return fail(GS_E_ARG, "null plan");
+        if (p->n_Z == 0) p->n_Z = 1;
+        propagator(p);
+        p->STEP_N++;
+        p->CURRENT_Z += p->ZSTEP;
+        return 0;
+    });
+}
+
+int gs_fixed_step(gs_plan* p, double dz, double* maxA2, double* maxNL2)
+{
+    return guarded([&]() {
+        if (!p) return fail(GS_E_ARG, "null plan");
+        double a, b;
+        nl_step(p, dz, &a, &b);
+        linear_half(p, dz);
+        hipEvent_t end;
+        HCK(hipEventCreate(&end));
+        HCK(hipEventRecord(end, p->st));
+        finish_timings(p, end);
+        (void)hipEventDestroy(end);
+        p->ZSTEP = dz;
+        p->CURRENT_Z += dz;
+        p->STEP_N++;
+        if (maxA2) *maxA2 = a;
+        if (maxNL2) *maxNL2 = b;
+        return 0;
+    });
+}
+
+int gs_march(gs_plan* p, const char* outdir, long max_steps)
+{
+    return guarded([&]() {
+        if (!p) return fail(GS_E_ARG, "null plan");
+        DumpSet ds;
+        open_dumps(p, outdir ? outdir : ".", &ds);
+        const auto& Z = p->s.ZNET;
+        long steps = 0;
+        for (;;) {
+            if (p->CURRENT_Z >= Z[p->n_Z]) {
+                do_dumps(p, &ds);
+                if (p->n_Z == p->s.NZ - 1) return 0;
+                p->n_Z++;
+            }
+            if (max_steps > 0 && steps >= max_steps) return 1;
+            if (p->n_Z == 0) throw gs::Error(GS_E_PHYSICS, "CURRENT_Z is below the first dump plane");
+            propagator(p);
+            p->STEP_N++;
+            p->CURRENT_Z += p->ZSTEP;
+            ++steps;
+        }
+    });
+}
+
+int gs_line_diagnostics(gs_plan* p, double* flux, double* maxI, double* rhomax)
+{
+    return guarded([&]() {
+        if (!p) return fail(GS_E_ARG, "null plan");
+        const int nr = p->s.NR;
+        double* d = p->diag.as<double>();
+        LCK(gs::launch_line_diag(p->prec, p->s.NT, p->field.p, nr, nullptr, p->s.TSTEP, p->nl, d, d + p->rp, d + 2 * p->rp,
+                                 nullptr, nullptr, p->st));
+        if (flux) HCK(hipMemcpyAsync(flux, d, nr * 8, hipMemcpyDeviceToHost, p->st));
+        if (maxI) HCK(hipMemcpyAsync(maxI, d + p->rp, nr * 8, hipMemcpyDeviceToHost, p->st));
+        if (rhomax) HCK(hipMemcpyAsync(rhomax, d + 2 * p->rp, nr * 8, hipMemcpyDeviceToHost, p->st));
+        check_status(p, nullptr);
+        return 0;
+    });
+}
+
+int gs_fft_lines(gs_plan* p, const double* in, double* out, int n_lines, int sign)
+{
+    return guarded([&]() {
+        if (!p || !in || !out || n_lines < 1) return fail(GS_E_ARG, "bad argument");
+        const size_t n = (size_t)n_lines * p->s.NT;
+        DevBuf d64, din, dout;
+        upload(d64, in, n * 16);
+        din.alloc(n * p->relem);
+        dout.alloc(n * p->relem);
+        LCK(gs::launch_convert(p->prec, d64.as<double>(), din.p, n, 1, p->st));
+        LCK(gs::launch_fft_lines(p->prec, p->s.NT, din.p, dout.p, n_lines, sign, 1.0, nullptr, p->tw.p, p->st));
+        LCK(gs::launch_convert(p->prec, d64.as<double>(), dout.p, n, 0, p->st));
+        HCK(hipMemcpyAsync(out, d64.p, n * 16, hipMemcpyDeviceToHost, p->st));
+        sync(p);
+        return 0;
+    });
+}
+
+int gs_last_timings(gs_plan* p, double* ms4)
+{
+    if (!p || !ms4) return fail(GS_E_ARG, "null argument");
+    for (int i = 0; i < 4; ++i) ms4[i] = p->timings[i];
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,170 @@
+"""HIP path (through the C-ABI) against the oracle on the same inputs, for
+the three precisions.  Tolerances: FP64 agrees with the FP64 oracle to
+rounding; FP32 and the split-BF16 mixed mode to the stated relative L2."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import write_cfg, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+gsa = pytest.importorskip("gsolver_amd")
+
+# (relative L2 bound for a single transform / RHS evaluation)
+TOL = {"fp64": 1e-11, "fp32": 2e-5, "mixed": 2e-5}
+HOT = dict(ENERGY="20e-6", DIAMETER_FWHM="4e-5", N_T="512", N_R="64", R_MAX="0.2e-3")
+
+
+def pair(tmp_path, precision, nthreads=8, **kw):
+    cfg = write_cfg(tmp_path / "c.txt", **kw)
+    return gsa.Plan(cfg, precision=precision), oracle.Sim(cfg, nthreads=nthreads)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+@pytest.mark.parametrize("nt", [256, 512, 1024, 2048, 4096])
+def test_fft_lines_vs_numpy(tmp_path, prec, nt):
+    p = gsa.Plan(write_cfg(tmp_path / "c.txt", N_T=nt, N_R=8), precision=prec)
+    rng = np.random.default_rng(nt)
+    x = rng.normal(size=(3, nt)) + 1j * rng.normal(size=(3, nt))
+    for sign in (-1, +1):
+        ref = np.fft.fft(x, axis=1) if sign < 0 else np.fft.ifft(x, axis=1) * nt
+        got = p.fft_lines(x, sign)
+        err = rel_l2(got, ref)
+        print("fft", prec, nt, sign, err)
+        assert err < (1e-14 if prec == "fp64" else 3e-7)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+def test_setup_tables_match_oracle(tmp_path, prec):
+    p, o = pair(tmp_path, prec, N_R=128, N_T=512)
+    for name in ("omega", "wavenum", "ho", "kerr", "plasma_func", "qdht_x", "qdht_m1", "znet"):
+        e = rel_l2(p.table(name), o.get(name))
+        print(name, e)
+        assert e < 1e-13, name
+    e = rel_l2(p.table("raman_function"), o.get("raman_function"))
+    assert e < 1e-13
+    C = p.table("qdht_c")
+    assert rel_l2(C, o.get("qdht_c")) < 1e-13
+    tab = p.table("ionrate_ln")
+    assert np.abs(tab - o.get("ionrate_ln")).max() < 1e-11
+    for s in ("ZSTEP", "GROUP_VELOCITY", "OMEGA_MAX", "OMEGA_MIN"):
+        assert abs(p.scalar(s) - o.scalar(s)) <= 1e-13 * abs(o.scalar(s)), s
+    f = p.get()
+    assert rel_l2(f, o.get("field")) < (1e-13 if prec == "fp64" else 1e-7)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+@pytest.mark.parametrize("nr", [64, 256])
+def test_qdht_vs_oracle(tmp_path, prec, nr):
+    p, o = pair(tmp_path, prec, N_R=nr, N_T=512)
+    rng = np.random.default_rng(nr)
+    X = (rng.normal(size=(nr, 512)) + 1j * rng.normal(size=(nr, 512)))
+    p.put(X, gsa.B1)
+    p.qdht(gsa.B1)
+    got = p.get(gsa.B1)
+    ref = o.qdht(X)
+    err = rel_l2(got, ref)
+    print("qdht", prec, nr, err)
+    assert err < TOL[prec]
+    # involution: a second application returns the input (to the QDHT's own accuracy)
+    p.qdht(gsa.B1)
+    assert rel_l2(p.get(gsa.B1), X) < max(TOL[prec], 1e-8)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+def test_b2_initial_qdht(tmp_path, prec):
+    p, o = pair(tmp_path, prec, N_R=128, N_T=512)
+    err = rel_l2(p.get(gsa.B2), o.get("b2"))
+    print("b2", prec, err)
+    assert err < (1e-12 if prec == "fp64" else 1e-7)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+def test_nl_response_vs_oracle(tmp_path, prec):
+    """calculate_NLresponse on every line of a field hot enough to ionize
+    (peak ~3.5e17 W/m^2, above the loss threshold)."""
+    p, o = pair(tmp_path, prec, **HOT)
+    F = o.get("field")
+    got = p.nl_response(F)
+    ref = np.array([o.nl_response(F[r]) for r in range(o.R)])
+    err = rel_l2(got, ref)
+    print("nl_response", prec, err, "max|E|^2", np.max(np.abs(np.fft.ifft(F, axis=1)) ** 2))
+    assert err < {"fp64": 1e-11, "fp32": 1e-4, "mixed": 1e-4}[prec]
+    # plasma part alone: rho(t) of the hottest line through the line diagnostics
+    flux, mI, rho = p.line_diagnostics()
+    ro = np.array([o.plasma(np.fft.ifft(F[r]))[0].max() for r in range(o.R)])
+    assert ro.max() > 0
+    e2 = rel_l2(rho, ro)
+    print("rhomax", prec, e2)
+    assert e2 < {"fp64": 1e-11, "fp32": 1e-4, "mixed": 1e-5}[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+def test_nl_step_vs_oracle(tmp_path, prec):
+    p, o = pair(tmp_path, prec, **HOT)
+    dz = 2e-6
+    o.set_scalar("ZSTEP", dz)
+    a_ref, b_ref = o.nl_step()
+    a, b = p.nl_step(dz)
+    err = rel_l2(p.get(gsa.B1), o.get("b1"))
+    print("nl_step", prec, err, a / a_ref - 1, b / b_ref - 1)
+    assert err < {"fp64": 1e-11, "fp32": 1e-4, "mixed": 1e-4}[prec]
+    assert abs(a / a_ref - 1) < {"fp64": 1e-13, "fp32": 1e-6, "mixed": 1e-6}[prec]
+    assert abs(b / b_ref - 1) < {"fp64": 1e-11, "fp32": 1e-4, "mixed": 1e-4}[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+def test_fixed_steps_vs_oracle(tmp_path, prec):
+    """Three full z-steps (NL RK4, QDHT, fused linear step, QDHT back) at a fixed dz."""
+    p, o = pair(tmp_path, prec, **HOT)
+    for k in range(3):
+        o.fixed_step(2e-6)
+        p.fixed_step(2e-6)
+    err = rel_l2(p.get(), o.get("field"))
+    e2 = rel_l2(p.get(gsa.B2), o.get("b2"))
+    print("fixed", prec, err, e2)
+    assert err < {"fp64": 1e-11, "fp32": 2e-5, "mixed": 2e-5}[prec]
+    assert e2 < {"fp64": 1e-11, "fp32": 2e-5, "mixed": 2e-5}[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+def test_linear_step_standalone(tmp_path, prec):
+    p, o = pair(tmp_path, prec, N_R=128, N_T=512)
+    rng = np.random.default_rng(1)
+    X = 1e-3 * (rng.normal(size=(o.R, o.T)) + 1j * rng.normal(size=(o.R, o.T)))
+    o.put("b1", X)
+    o.set_scalar("ZSTEP", 5e-6)
+    o.lstep()
+    p.put(X, gsa.B1)
+    p.linear_step(5e-6)
+    err = rel_l2(p.get(gsa.FK), o.get("field"))
+    print("lstep", prec, err)
+    assert err < {"fp64": 1e-13, "fp32": 1e-5, "mixed": 1e-6}[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+def test_adaptive_march_vs_oracle(tmp_path, prec):
+    """The reference controller over 3 dump planes of a hot small run: fluence,
+    on-axis field and plasma maxima at the planes (.udf outputs)."""
+    kw = dict(HOT, N_Z="3", Z_MAX="1e-4", MAX_TOLERANCE="3e-3", DUMPS_N="3", DUMPTYPE0="flux",
+              DUMP_FILENAME0="g_%s_f.udf", DUMPTYPE1="plasma_max", DUMP_FILENAME1="g_%s_pm.udf",
+              DUMPTYPE2="field_axis", DUMP_FILENAME2="g_%s_fa.udf")
+    cfg = write_cfg(tmp_path / "c.txt", **kw)
+    gdir = tmp_path / "gpu"
+    odir = tmp_path / "orc"
+    gdir.mkdir()
+    odir.mkdir()
+    p = gsa.Plan(cfg, precision=prec)
+    o = oracle.Sim(cfg, nthreads=8)
+    assert p.march(str(gdir)) == 0
+    o.run(str(odir))
+    for suf, tol in (("f", 1e-4), ("pm", 1e-3), ("fa", 1e-4)):
+        hg, g = oracle.load_udf(str(gdir / ("g__%s.udf" % suf)))
+        ho, r = oracle.load_udf(str(odir / ("g__%s.udf" % suf)))
+        assert (hg == ho).all()
+        errs = [rel_l2(g[z], r[z]) for z in range(g.shape[0])]
+        print("march", prec, suf, errs, p.scalar("STEP_N"), o.scalar("STEP_N"))
+        assert max(errs) < (tol if prec != "fp64" else 1e-9)

@@ -1,0 +1,71 @@
+"""The product's host-side set-up (C++, gs_setup_table: no GPU needed)
+against the oracle's restatement, and the C-ABI export list.  CPU only."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import ROOT, write_cfg, rel_l2
+
+gsa = pytest.importorskip("gsolver_amd")
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "gsolver_amd.h")).read()
+    names = re.findall(r"^\s*(?:int|void|const char\*)\s+(gs_\w+)\s*\(", hdr, re.M)
+    assert len(names) >= 20
+    L = gsa.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("kw", [dict(N_R=64, N_T=256), dict(N_R=256, N_T=1024, R_MAX="0.5e-3", T_MIN="-200e-15",
+                                                              T_MAX="200e-15")])
+def test_setup_tables_match_oracle(tmp_path, kw):
+    cfg = write_cfg(tmp_path / "c.txt", **kw)
+    o = oracle.Sim(cfg, nthreads=4)
+    R, T = o.R, o.T
+    for name, n, cplx, tol in [("omega", T, False, 0), ("wavenum", T, True, 1e-15), ("ho", T, False, 1e-15),
+                               ("kerr", T, False, 1e-15), ("plasma_func", T, True, 1e-15),
+                               ("raman_function", T, True, 1e-13), ("qdht_x", R, False, 1e-14),
+                               ("qdht_m1", R, False, 1e-14), ("znet", o.NZ, False, 0)]:
+        got = gsa.setup_table(cfg, name, n, cplx)
+        e = rel_l2(got, o.get(name))
+        assert e <= tol, (name, e)
+    C = gsa.setup_table(cfg, "qdht_c", R * R).reshape(R, R)
+    assert rel_l2(C, o.get("qdht_c")) < 2e-14
+    ion = gsa.setup_table(cfg, "ionrate_ln", 8192)
+    assert np.abs(ion - o.get("ionrate_ln")).max() < 1e-12
+
+
+def test_setup_errors_carry_reference_messages(tmp_path):
+    cfg = write_cfg(tmp_path / "c.txt")
+    with pytest.raises(gsa.GsError, match="parameter N_Z not found"):
+        txt = open(cfg).read().replace("N_Z 3\n", "")
+        open(cfg, "w").write(txt)
+        gsa.setup_table(cfg, "omega", 256)
+    cfg2 = write_cfg(tmp_path / "d.txt", NOISE_LEVEL="0.1")
+    with pytest.raises(gsa.GsError, match="NOISE_MAP_LEVEL"):
+        gsa.setup_table(cfg2, "omega", 256)
+
+
+def test_overrides_take_precedence(tmp_path):
+    cfg = write_cfg(tmp_path / "c.txt")
+    a = gsa.setup_table(cfg, "omega", 256)
+    b = gsa.setup_table(cfg, "omega", 256, overrides=["T_MAX", "200e-15", "T_MIN", "-200e-15"])
+    assert abs((b[1] - b[0]) / (a[1] - a[0]) - 0.5) < 1e-12
+
+
+def test_noise_map_is_seeded_and_matches_oracle(tmp_path):
+    """The NOISE_MAP_LEVEL/NOISE_SEED amplitude map is identical in both set-ups
+    (checked through the oracle's initial field, whose spectrum depends on it)."""
+    cfg = write_cfg(tmp_path / "c.txt", NOISE_MAP_LEVEL="0.01", NOISE_SEED="12345")
+    o1 = oracle.Sim(cfg)
+    o2 = oracle.Sim(cfg)
+    assert np.array_equal(o1.get("field"), o2.get("field"))
+    cfg0 = write_cfg(tmp_path / "d.txt")
+    f0 = oracle.Sim(cfg0).get("field")
+    d = rel_l2(o1.get("field"), f0)
+    assert 1e-3 < d < 2e-2
