@@ -1,0 +1,306 @@
+// gs_dev.h -- device-side building blocks shared by the line kernels
+// (gs_line.hip) and the QDHT / linear-step kernels (gs_qdht.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "gs_device.h"
+
+namespace gs {
+
+// ------------------------------------------------------------------ types
+template <class R>
+struct alignas(2 * sizeof(R)) C2 {
+    R x, y;
+};
+
+template <class R> __device__ __forceinline__ C2<R> cadd(C2<R> a, C2<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <class R> __device__ __forceinline__ C2<R> csub(C2<R> a, C2<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <class R> __device__ __forceinline__ C2<R> cmul(C2<R> a, C2<R> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+template <class R> __device__ __forceinline__ C2<R> cscale(C2<R> a, R s) { return {a.x * s, a.y * s}; }
+// multiply by sign*i
+template <int SIGN, class R> __device__ __forceinline__ C2<R> cmuli(C2<R> a) { return SIGN > 0 ? C2<R>{-a.y, a.x} : C2<R>{a.y, -a.x}; }
+
+struct PolF64 { using R = double; using S = double; using W = double; using P = double; };
+struct PolF32 { using R = float;  using S = float;  using W = float;  using P = float;  };
+struct PolMix { using R = float;  using S = double; using W = float;  using P = double; };
+
+
+#define HIPCHK(x)                                  \
+    do {                                           \
+        hipError_t e_ = (x);                       \
+        if (e_ != hipSuccess) return (int)e_;      \
+    } while (0)
+
+// ------------------------------------------------------------------ small DFTs
+// X[q] = sum_r x[r] e^{SIGN 2 pi i r q / RADIX}
+template <int RADIX, int SIGN, class R>
+__device__ __forceinline__ void dft_small(C2<R>* v)
+{
+    if constexpr (RADIX == 2) {
+        C2<R> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (RADIX == 4) {
+        C2<R> s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        C2<R> s13 = cadd(v[1], v[3]), d13 = cmuli<SIGN>(csub(v[1], v[3]));
+        v[0] = cadd(s02, s13);
+        v[1] = cadd(d02, d13);
+        v[2] = csub(s02, s13);
+        v[3] = csub(d02, d13);
+    } else if constexpr (RADIX == 8) {
+        const R h = (R)0.70710678118654752440084436210485;
+        C2<R> a[4], b[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            a[r] = cadd(v[r], v[r + 4]);
+            b[r] = csub(v[r], v[r + 4]);
+        }
+        // b[r] *= w8^r, w8 = e^{SIGN i pi/4}
+        b[1] = C2<R>{h * (b[1].x - (R)SIGN * b[1].y), h * (b[1].y + (R)SIGN * b[1].x)};
+        b[2] = cmuli<SIGN>(b[2]);
+        b[3] = C2<R>{h * (-b[3].x - (R)SIGN * b[3].y), h * (-b[3].y + (R)SIGN * b[3].x)};
+        dft_small<4, SIGN>(a);
+        dft_small<4, SIGN>(b);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            v[2 * q] = a[q];
+            v[2 * q + 1] = b[q];
+        }
+    } else {
+        static_assert(RADIX == 16, "radix");
+        // 16 = 4 x 4: r = r1 + 4 r2, q = q2 + 4 q1; inner radix-4 over r2, twiddle w16^{r1 q2}, outer over r1
+        const R h = (R)0.70710678118654752440084436210485;
+        const R c1 = (R)0.92387953251128675612818318939679, s1 = (R)0.38268343236508977172845998403040;
+        const R sg = (R)SIGN;
+        C2<R> y[4][4];
+#pragma unroll
+        for (int r1 = 0; r1 < 4; ++r1) {
+            C2<R> t[4] = {v[r1], v[r1 + 4], v[r1 + 8], v[r1 + 12]};
+            dft_small<4, SIGN>(t);
+#pragma unroll
+            for (int q2 = 0; q2 < 4; ++q2) y[r1][q2] = t[q2];
+        }
+        const C2<R> w1 = {c1, sg * s1}, w2 = {h, sg * h}, w3 = {s1, sg * c1}, w6 = {-h, sg * h}, w9 = {-c1, -sg * s1};
+        y[1][1] = cmul(y[1][1], w1);
+        y[1][2] = cmul(y[1][2], w2);
+        y[1][3] = cmul(y[1][3], w3);
+        y[2][1] = cmul(y[2][1], w2);
+        y[2][2] = cmuli<SIGN>(y[2][2]);
+        y[2][3] = cmul(y[2][3], w6);
+        y[3][1] = cmul(y[3][1], w3);
+        y[3][2] = cmul(y[3][2], w6);
+        y[3][3] = cmul(y[3][3], w9);
+#pragma unroll
+        for (int q2 = 0; q2 < 4; ++q2) {
+            C2<R> t[4] = {y[0][q2], y[1][q2], y[2][q2], y[3][q2]};
+            dft_small<4, SIGN>(t);
+#pragma unroll
+            for (int q1 = 0; q1 < 4; ++q1) v[q2 + 4 * q1] = t[q1];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ block scans
+template <class V> struct AffR { V a, b; };               // x -> a x + b
+template <class V> struct AffC { V ar, ai, br, bi; };     // complex x -> a x + b
+
+template <class V> __device__ __forceinline__ AffR<V> compose(AffR<V> e, AffR<V> l)
+{   // apply e then l
+    return {l.a * e.a, l.a * e.b + l.b};
+}
+template <class V> __device__ __forceinline__ AffC<V> compose(AffC<V> e, AffC<V> l)
+{
+    return {l.ar * e.ar - l.ai * e.ai, l.ar * e.ai + l.ai * e.ar,
+            l.ar * e.br - l.ai * e.bi + l.br, l.ar * e.bi + l.ai * e.br + l.bi};
+}
+template <class V> __device__ __forceinline__ AffR<V> shfl_up(AffR<V> v, int d) { return {__shfl_up(v.a, d), __shfl_up(v.b, d)}; }
+template <class V> __device__ __forceinline__ AffC<V> shfl_up(AffC<V> v, int d)
+{
+    return {__shfl_up(v.ar, d), __shfl_up(v.ai, d), __shfl_up(v.br, d), __shfl_up(v.bi, d)};
+}
+template <class V> __device__ __forceinline__ AffR<V> aff_id(AffR<V>*) { return {(V)1, (V)0}; }
+template <class V> __device__ __forceinline__ AffC<V> aff_id(AffC<V>*) { return {(V)1, (V)0, (V)0, (V)0}; }
+
+// Exclusive block scan of affine maps (composition, earlier threads first).
+// lds must hold NW+1 entries.  Returns the prefix; *total = all threads composed.
+template <int NT, class A>
+__device__ A block_scan_excl(A v, A* lds, A* total)
+{
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const A id = aff_id((A*)nullptr);
+    A inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        A o = shfl_up(inc, d);
+        if (lane >= d) inc = compose(o, inc);
+    }
+    if (lane == 63) lds[wid] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        A run = id;
+        for (int w = 0; w < NW; ++w) {
+            A t = lds[w];
+            lds[w] = run;
+            run = compose(run, t);
+        }
+        lds[NW] = run;
+    }
+    __syncthreads();
+    A exc = shfl_up(inc, 1);
+    if (lane == 0) exc = id;
+    A res = compose(lds[wid], exc);
+    *total = lds[NW];
+    __syncthreads();
+    return res;
+}
+
+template <int NT>
+__device__ double block_max(double v, double* lds)
+{
+    constexpr int NW = NT / 64;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double r = lds[0];
+    for (int w = 1; w < NW; ++w) r = fmax(r, lds[w]);
+    __syncthreads();
+    return r;
+}
+
+// block-wide max of two values at once (one barrier pair); lds: 2*NW doubles
+template <int NT>
+__device__ void block_max2(double& a, double& b, double* lds)
+{
+    constexpr int NW = NT / 64;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        a = fmax(a, __shfl_xor(a, d));
+        b = fmax(b, __shfl_xor(b, d));
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) { lds[wid] = a; lds[NW + wid] = b; }
+    __syncthreads();
+    a = lds[0];
+    b = lds[NW];
+    for (int w = 1; w < NW; ++w) { a = fmax(a, lds[w]); b = fmax(b, lds[NW + w]); }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------ plasma
+// W(I), ionization.cpp:98-147 (solid, table branch).  Sets status bits on
+// the reference's throw conditions and returns 0 there.
+template <class W>
+__device__ __forceinline__ W photoionization(W I, const double* __restrict__ tab, int* flags)
+{
+    if (I < (W)1e-3) return 0;   // IONIZATION_MIN_I / INTENSITY_DENOM
+    const W lnI = log(I) + (W)36.841361487904734;
+    if (lnI != lnI) { *flags |= kFlagNaN; return 0; }
+    const W u = (lnI - (W)29.933606208922594) / (W)0.003304003148325;
+    const int n = (int)floor(u);
+    if (n > 8190) { *flags |= kFlagBlast; return 0; }
+    if (n < 0) return 0;
+    const W lnIn = (W)29.933606208922594 + (W)n * (W)0.003304003148325;
+    const W a = (W)tab[n], b = (W)tab[n + 1];
+    return exp(a + (lnI - lnIn) * (b - a) / (W)0.003304003148325);
+}
+
+// One Heun step of ionization.cpp:36-41 from rho = x with the rates of the
+// previous sample (Wp, cp) and of this sample (Wn, cn); returns the new rho
+// and the derivative d(new)/dx (0 where the clamp at N is active).
+template <class S>
+__device__ __forceinline__ S heun_step(S x, S Wp, S cp, S Wn, S cn, S dt, S N, S invN, S invtau, S* deriv)
+{
+    const S W01 = x > N ? (S)0 : Wp;
+    const S av0 = (x < 0 || x > N) ? (S)0 : cp * x * ((S)1 - x * invN);
+    const S re0 = x > 0 ? x * invtau : (S)0;
+    const S k1 = dt * (W01 + av0 - re0);
+    const S xm = x + k1;
+    const S W11 = xm > N ? (S)0 : Wn;
+    const S av1 = (xm < 0 || xm > N) ? (S)0 : cn * xm * ((S)1 - xm * invN);
+    const S re1 = xm > 0 ? xm * invtau : (S)0;
+    const S k2 = dt * (W11 + av1 - re1);
+    S y = x + (S)0.5 * (k1 + k2);
+    // derivative of the active branch (x >= 0 side at the kink of rec)
+    const S dav0 = (x < 0 || x > N) ? (S)0 : cp * ((S)1 - (S)2 * x * invN);
+    const S dre0 = x >= 0 ? invtau : (S)0;
+    const S dk1 = dt * (dav0 - dre0);
+    const S dav1 = (xm < 0 || xm > N) ? (S)0 : cn * ((S)1 - (S)2 * xm * invN);
+    const S dre1 = xm >= 0 ? invtau : (S)0;
+    const S dk2 = dt * (dav1 - dre1) * ((S)1 + dk1);
+    S d = (S)1 + (S)0.5 * (dk1 + dk2);
+    if (y > N) { y = N; d = 0; }
+    *deriv = d;
+    return y;
+}
+
+// ------------------------------------------------------------------ linear step
+__device__ __forceinline__ double sqrtho_d(double x)
+{
+    // sqrt(1+x)-1 as the 20-term Taylor sum of extmath.h:245-252, Horner form
+    const double c[20] = {0.5, -0.125, 0.0625, -0.0390625, 0.02734375, -0.0205078125, 0.01611328125,
+                          -0.013092041015625, 0.010910034179688, -0.009273529052734, 0.008008956909180,
+                          -0.007007837295532, 0.006199240684509, -0.005535036325455, 0.004981532692909,
+                          -0.004514514002949, 0.004116174532101, -0.003773159987759, 0.003475278936094,
+                          -0.003214633015887};
+    double r = c[19];
+#pragma unroll
+    for (int j = 18; j >= 0; --j) r = r * x + c[j];
+    return r * x;
+}
+__device__ __forceinline__ float sqrtho_f(float x)
+{
+    const float c[20] = {0.5f, -0.125f, 0.0625f, -0.0390625f, 0.02734375f, -0.0205078125f, 0.01611328125f,
+                         -0.013092041015625f, 0.010910034179688f, -0.009273529052734f, 0.008008956909180f,
+                         -0.007007837295532f, 0.006199240684509f, -0.005535036325455f, 0.004981532692909f,
+                         -0.004514514002949f, 0.004116174532101f, -0.003773159987759f, 0.003475278936094f,
+                         -0.003214633015887f};
+    float r = c[19];
+#pragma unroll
+    for (int j = 18; j >= 0; --j) r = r * x + c[j];
+    return r * x;
+}
+
+template <class Pol>
+__device__ __forceinline__ void lstep_point(int i, int w, size_t o, typename Pol::R yr, typename Pol::R yi, const LstepArgs& la)
+{
+    using R = typename Pol::R;
+    using P = typename Pol::P;
+    C2<R>* fk = (C2<R>*)la.fk;
+    C2<R>* b2 = (C2<R>*)la.b2;
+    if (i >= la.nr || !la.wok[w]) { fk[o] = C2<R>{0, 0}; return; }
+    const P kt2 = (P)la.kt2[i];
+    if (kt2 > (P)la.lim[w]) { fk[o] = C2<R>{0, 0}; return; }
+    const P k1 = (P)la.k1[w];
+    const P s = sizeof(P) == 8 ? (P)sqrtho_d((double)(-kt2 / k1 / k1)) : (P)sqrtho_f((float)(-kt2 / k1 / k1));
+    const P dz = (P)la.dz;
+    const C2<R> b = b2[o];
+    const P br = (P)b.x + dz * (P)yr / ((P)1 + s);
+    const P bi = (P)b.y + dz * (P)yi / ((P)1 + s);
+    const P ph = dz * (-(P)la.ho[w] - s * k1);
+    const P amp = exp(dz * (P)la.k2[w]);
+    P sn, cs;
+    if constexpr (sizeof(P) == 8) sincos(ph, &sn, &cs);
+    else sincosf(ph, &sn, &cs);
+    const C2<R> f = {(R)(amp * (br * cs - bi * sn)), (R)(amp * (br * sn + bi * cs))};
+    fk[o] = f;
+    b2[o] = f;
+}
+
+template <class Pol>
+__global__ void k_lstep(const C2<typename Pol::R>* __restrict__ b1k, int nt, size_t n, LstepArgs la)
+{
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    const int i = (int)(o / nt), w = (int)(o % nt);
+    const C2<typename Pol::R> y = b1k[o];
+    lstep_point<Pol>(i, w, o, y.x, y.y, la);
+}
+
+}  // namespace gs

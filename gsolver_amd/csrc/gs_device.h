@@ -1,6 +1,7 @@
 // gs_device.h -- device-side data of a plan and the launchers of the gfx950
-// kernels in gs_device.hip.  Host code (gs_plan.cpp) fills these structs and
-// calls the launchers; nothing here is visible through the C-ABI.
+// kernels (line kernels in gs_line.hip, QDHT GEMMs and element-wise kernels in
+// gs_qdht.hip).  Host code (gs_plan.cpp) fills these structs and calls the
+// launchers; nothing here is visible through the C-ABI.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -42,6 +43,7 @@ struct NLArgs {
     const void* kerr;           // KERR_PROFILE (R)
     const void* tw;             // e^{-2 pi i k/T}, k < T (C2<R>)
     DevStatus* status;
+    int nl_e;                   // samples per thread of the RK4 kernel (0: default)
 };
 
 struct LstepArgs {

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -315,6 +316,9 @@ void build_plan(gs_plan* p)
     a.kerr = p->kerr.p;
     a.tw = p->tw.p;
     a.status = p->status.as<gs::DevStatus>();
+    // samples per thread of the RK4 line kernel (tuning knob; 0 = built-in default)
+    a.nl_e = 0;
+    if (const char* e = std::getenv("GS_NL_E")) a.nl_e = std::atoi(e);
     gs::LstepArgs& l = p->ls;
     l.nr = nr;
     l.kt2 = p->kt2.as<double>();

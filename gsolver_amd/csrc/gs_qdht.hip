@@ -1,0 +1,409 @@
+// gs_qdht.hip -- the QDHT (fhatha.cpp:218-230) as a real GEMM on MFMA with
+// the linear step (propagator.cu:570-625) fused into the epilogue, the
+// stand-alone linear step, and small helper kernels (reductions, precision
+// conversions, average spectrum).
+//
+//  Y[R][2T] = A[R][R] X[R][2T], A = diag(m1) C diag(1/m1), R padded to the
+//  tile.  Complex columns are split into re/im planes when staged into LDS,
+//  so one lane ends up owning both parts of an output and can apply the
+//  linear step in registers.
+//   FP32   v_mfma_f32_32x32x2_f32, tile 128 x 64 complex, K step 16
+//   MIXED  split-BF16 3-pass (hi*hi + hi*lo + lo*hi, FP32 accumulate),
+//          v_mfma_f32_32x32x16_bf16, tile 128 x 64 complex, K step 32
+//   FP64   v_mfma_f64_16x16x4_f64, tile 64 x 32 complex, K step 16
+#include "gs_dev.h"
+
+namespace gs {
+
+// ------------------------------------------------------------------ QDHT GEMMs
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// blockIdx -> tile, grouping all row tiles of a column tile on one XCD so the
+// B columns are reused from that XCD's L2 (dispatch is round-robin over 8 XCDs)
+__device__ __forceinline__ void tile_of(int mt, int ntl, int* mi, int* ni)
+{
+    const int tiles = mt * ntl, b = blockIdx.x;
+    int t = b;
+    if ((tiles & 7) == 0) t = (b & 7) * (tiles >> 3) + (b >> 3);
+    *mi = t % mt;
+    *ni = t / mt;
+}
+
+// FP32: tile 128 rows x 64 complex columns, K step 16, 4 waves (2x2)
+template <bool LSTEP>
+__global__ __launch_bounds__(256) void k_qdht_f32(const float* __restrict__ At, const C2<float>* __restrict__ X,
+                                                  C2<float>* __restrict__ Y, int K, int NC, LstepArgs la)
+{
+    constexpr int BM = 128, BNC = 64, BK = 16, PA = 32, PB = 32;
+    __shared__ float As[BK][BM + PA];
+    __shared__ float Br[BK][BNC + PB];
+    __shared__ float Bi[BK][BNC + PB];
+    const int M = K;
+    int mi, ni;
+    tile_of(M / BM, NC / BNC, &mi, &ni);
+    const int m0 = mi * BM, n0 = ni * BNC;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = (w >> 1) * 64, wn = (w & 1) * 32;
+    const int lr = tid >> 4, lc = (tid & 15);
+
+    v16f acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][c][r] = 0.f;
+
+    float4 ra0, ra1, rb0, rb1;
+    auto gload = [&](int k0) {
+        const float* pa = At + (size_t)(k0 + lr) * M + m0 + lc * 8;
+        ra0 = *(const float4*)pa;
+        ra1 = *(const float4*)(pa + 4);
+        const float* pb = (const float*)(X + (size_t)(k0 + lr) * NC + n0 + lc * 4);
+        rb0 = *(const float4*)pb;
+        rb1 = *(const float4*)(pb + 4);
+    };
+    auto sstore = [&]() {
+        *(float4*)&As[lr][lc * 8] = ra0;
+        *(float4*)&As[lr][lc * 8 + 4] = ra1;
+        Br[lr][lc * 4 + 0] = rb0.x; Bi[lr][lc * 4 + 0] = rb0.y;
+        Br[lr][lc * 4 + 1] = rb0.z; Bi[lr][lc * 4 + 1] = rb0.w;
+        Br[lr][lc * 4 + 2] = rb1.x; Bi[lr][lc * 4 + 2] = rb1.y;
+        Br[lr][lc * 4 + 3] = rb1.z; Bi[lr][lc * 4 + 3] = rb1.w;
+    };
+    gload(0);
+    sstore();
+    __syncthreads();
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        const bool more = k0 + BK < K;
+        if (more) gload(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 2) {
+            const int kr = kk + (lane >> 5), cl = lane & 31;
+            const float a0 = As[kr][wm + cl], a1 = As[kr][wm + 32 + cl];
+            const float br = Br[kr][wn + cl], bi = Bi[kr][wn + cl];
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, br, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, bi, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, br, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bi, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+        if (more) {
+            sstore();
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = m0 + wm + rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int col = n0 + wn + (lane & 31);
+            const size_t o = (size_t)row * NC + col;
+            if (LSTEP) lstep_point<PolF32>(row, col, o, acc[rb][0][r], acc[rb][1][r], la);
+            else Y[o] = C2<float>{acc[rb][0][r], acc[rb][1][r]};
+        }
+}
+
+// BF16 3-pass: same tile, K step 32, operands k-contiguous in LDS
+__device__ __forceinline__ uint32_t bf16_rn(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+
+template <bool LSTEP>
+__global__ __launch_bounds__(256) void k_qdht_bf16x3(const uint16_t* __restrict__ Ah, const uint16_t* __restrict__ Al,
+                                                     const C2<float>* __restrict__ X, C2<float>* __restrict__ Y, int K,
+                                                     int NC, LstepArgs la)
+{
+    constexpr int BM = 128, BNC = 64, BK = 32, LD = BK + 8;
+    __shared__ __align__(16) uint16_t sAh[BM][LD];
+    __shared__ __align__(16) uint16_t sAl[BM][LD];
+    __shared__ __align__(16) uint16_t sBh[2][BNC][LD];
+    __shared__ __align__(16) uint16_t sBl[2][BNC][LD];
+    const int M = K;
+    int mi, ni;
+    tile_of(M / BM, NC / BNC, &mi, &ni);
+    const int m0 = mi * BM, n0 = ni * BNC;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = (w >> 1) * 64, wn = (w & 1) * 32;
+    // A loader: row tid/2, k half (tid&1)*16; B loader: k pair tid/16, 4 columns (tid&15)*4
+    const int arow = tid >> 1, ak = (tid & 1) * 16;
+    const int bkp = tid >> 4, bc = (tid & 15) * 4;
+
+    v16f acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][c][r] = 0.f;
+
+    uint4 rah0, rah1, ral0, ral1;
+    float4 x0a, x0b, x1a, x1b;
+    auto gload = [&](int k0) {
+        const size_t ao = (size_t)(m0 + arow) * K + k0 + ak;
+        rah0 = *(const uint4*)(Ah + ao);
+        rah1 = *(const uint4*)(Ah + ao + 8);
+        ral0 = *(const uint4*)(Al + ao);
+        ral1 = *(const uint4*)(Al + ao + 8);
+        const float* p0 = (const float*)(X + (size_t)(k0 + 2 * bkp) * NC + n0 + bc);
+        const float* p1 = (const float*)(X + (size_t)(k0 + 2 * bkp + 1) * NC + n0 + bc);
+        x0a = *(const float4*)p0;
+        x0b = *(const float4*)(p0 + 4);
+        x1a = *(const float4*)p1;
+        x1b = *(const float4*)(p1 + 4);
+    };
+    auto split_store = [&](int c, int part, float v0, float v1) {
+        const uint32_t h0 = bf16_rn(v0), h1 = bf16_rn(v1);
+        const uint32_t l0 = bf16_rn(v0 - __uint_as_float(h0 << 16)), l1 = bf16_rn(v1 - __uint_as_float(h1 << 16));
+        *(uint32_t*)&sBh[part][c][2 * bkp] = h0 | (h1 << 16);
+        *(uint32_t*)&sBl[part][c][2 * bkp] = l0 | (l1 << 16);
+    };
+    auto sstore = [&]() {
+        *(uint4*)&sAh[arow][ak] = rah0;
+        *(uint4*)&sAh[arow][ak + 8] = rah1;
+        *(uint4*)&sAl[arow][ak] = ral0;
+        *(uint4*)&sAl[arow][ak + 8] = ral1;
+        split_store(bc + 0, 0, x0a.x, x1a.x); split_store(bc + 0, 1, x0a.y, x1a.y);
+        split_store(bc + 1, 0, x0a.z, x1a.z); split_store(bc + 1, 1, x0a.w, x1a.w);
+        split_store(bc + 2, 0, x0b.x, x1b.x); split_store(bc + 2, 1, x0b.y, x1b.y);
+        split_store(bc + 3, 0, x0b.z, x1b.z); split_store(bc + 3, 1, x0b.w, x1b.w);
+    };
+    gload(0);
+    sstore();
+    __syncthreads();
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        const bool more = k0 + BK < K;
+        if (more) gload(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            const int ko = kk + 8 * (lane >> 5), cl = lane & 31;
+            v8bf ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb) {
+                ah[rb] = *(const v8bf*)&sAh[wm + rb * 32 + cl][ko];
+                al[rb] = *(const v8bf*)&sAl[wm + rb * 32 + cl][ko];
+            }
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                bh[c] = *(const v8bf*)&sBh[c][wn + cl][ko];
+                bl[c] = *(const v8bf*)&sBl[c][wn + cl][ko];
+            }
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    acc[rb][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[rb], bh[c], acc[rb][c], 0, 0, 0);
+                    acc[rb][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[rb], bl[c], acc[rb][c], 0, 0, 0);
+                    acc[rb][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[rb], bh[c], acc[rb][c], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+        if (more) {
+            sstore();
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = m0 + wm + rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int col = n0 + wn + (lane & 31);
+            const size_t o = (size_t)row * NC + col;
+            if (LSTEP) lstep_point<PolMix>(row, col, o, acc[rb][0][r], acc[rb][1][r], la);
+            else Y[o] = C2<float>{acc[rb][0][r], acc[rb][1][r]};
+        }
+}
+
+// FP64: tile 64 rows x 32 complex columns, K step 16, 4 waves (2x2)
+template <bool LSTEP>
+__global__ __launch_bounds__(256) void k_qdht_f64(const double* __restrict__ At, const C2<double>* __restrict__ X,
+                                                  C2<double>* __restrict__ Y, int K, int NC, LstepArgs la)
+{
+    constexpr int BM = 64, BNC = 32, BK = 16, PA = 8, PB = 8;
+    __shared__ double As[BK][BM + PA];
+    __shared__ double Br[BK][BNC + PB];
+    __shared__ double Bi[BK][BNC + PB];
+    const int M = K;
+    int mi, ni;
+    tile_of(M / BM, NC / BNC, &mi, &ni);
+    const int m0 = mi * BM, n0 = ni * BNC;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 16;
+    const int lr = tid >> 4, lc = tid & 15;
+    v4d acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[a][c][r] = 0.0;
+    double2 ra0, ra1, rb0, rb1;
+    auto gload = [&](int k0) {
+        const double* pa = At + (size_t)(k0 + lr) * M + m0 + lc * 4;
+        ra0 = *(const double2*)pa;
+        ra1 = *(const double2*)(pa + 2);
+        const double* pb = (const double*)(X + (size_t)(k0 + lr) * NC + n0 + lc * 2);
+        rb0 = *(const double2*)pb;
+        rb1 = *(const double2*)(pb + 2);
+    };
+    auto sstore = [&]() {
+        As[lr][lc * 4 + 0] = ra0.x; As[lr][lc * 4 + 1] = ra0.y;
+        As[lr][lc * 4 + 2] = ra1.x; As[lr][lc * 4 + 3] = ra1.y;
+        Br[lr][lc * 2 + 0] = rb0.x; Bi[lr][lc * 2 + 0] = rb0.y;
+        Br[lr][lc * 2 + 1] = rb1.x; Bi[lr][lc * 2 + 1] = rb1.y;
+    };
+    gload(0);
+    sstore();
+    __syncthreads();
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        const bool more = k0 + BK < K;
+        if (more) gload(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int kr = kk + (lane >> 4), cl = lane & 15;
+            const double a0 = As[kr][wm + cl], a1 = As[kr][wm + 16 + cl];
+            const double br = Br[kr][wn + cl], bi = Bi[kr][wn + cl];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, br, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bi, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, br, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bi, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+        if (more) {
+            sstore();
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            // v_mfma_f64_16x16x4_f64 result layout: row = lane/16 + 4 r, column = lane%16
+            const int row = m0 + wm + rb * 16 + (lane >> 4) + 4 * r;
+            const int col = n0 + wn + (lane & 15);
+            const size_t o = (size_t)row * NC + col;
+            if (LSTEP) lstep_point<PolF64>(row, col, o, acc[rb][0][r], acc[rb][1][r], la);
+            else Y[o] = C2<double>{acc[rb][0][r], acc[rb][1][r]};
+        }
+}
+
+// ------------------------------------------------------------------ small kernels
+__global__ void k_reduce_max(int nr, const double* __restrict__ lm, DevStatus* st)
+{
+    __shared__ double red[64];
+    double a = 0, b = 0;
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) {
+        a = fmax(a, lm[2 * i]);
+        b = fmax(b, lm[2 * i + 1]);
+    }
+    const double A = block_max<1024>(a, red);
+    const double B = block_max<1024>(b, red + 32);
+    if (threadIdx.x == 0) {
+        st->maxA2 = A;
+        st->maxNL2 = B;
+    }
+}
+
+template <class R>
+__global__ void k_convert(const double* __restrict__ src, C2<R>* __restrict__ dst, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = C2<R>{(R)src[2 * i], (R)src[2 * i + 1]};
+}
+template <class R>
+__global__ void k_convert_back(const C2<R>* __restrict__ src, double* __restrict__ dst, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        dst[2 * i] = (double)src[i].x;
+        dst[2 * i + 1] = (double)src[i].y;
+    }
+}
+
+template <class R>
+__global__ void k_avg_spectrum(int nr, int nt, const C2<R>* __restrict__ f, const double* __restrict__ wts, double* out)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nt) return;
+    double acc = 0;
+    for (int r = 0; r < nr; ++r) {
+        const C2<R> v = f[(size_t)r * nt + w];
+        acc += ((double)v.x * v.x + (double)v.y * v.y) * wts[r];
+    }
+    out[w] = acc;
+}
+
+int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* amat_lo, const void* in, void* out,
+                const LstepArgs* ls, hipStream_t st)
+{
+    LstepArgs la = ls ? *ls : LstepArgs{};
+    if (p == kFP64) {
+        if (rp % 64 || nt % 32) return -1001;
+        const dim3 grid((rp / 64) * (nt / 32));
+        if (ls) hipLaunchKernelGGL(k_qdht_f64<true>, grid, dim3(256), 0, st, (const double*)amat, (const C2<double>*)in, (C2<double>*)out, rp, nt, la);
+        else hipLaunchKernelGGL(k_qdht_f64<false>, grid, dim3(256), 0, st, (const double*)amat, (const C2<double>*)in, (C2<double>*)out, rp, nt, la);
+    } else if (p == kFP32) {
+        if (rp % 128 || nt % 64) return -1001;
+        const dim3 grid((rp / 128) * (nt / 64));
+        if (ls) hipLaunchKernelGGL(k_qdht_f32<true>, grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
+        else hipLaunchKernelGGL(k_qdht_f32<false>, grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
+    } else {
+        if (rp % 128 || nt % 64) return -1001;
+        const dim3 grid((rp / 128) * (nt / 64));
+        if (ls) hipLaunchKernelGGL(k_qdht_bf16x3<true>, grid, dim3(256), 0, st, (const uint16_t*)amat, (const uint16_t*)amat_lo, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
+        else hipLaunchKernelGGL(k_qdht_bf16x3<false>, grid, dim3(256), 0, st, (const uint16_t*)amat, (const uint16_t*)amat_lo, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int launch_lstep(Prec p, int rp, int nt, const void* b1k, const LstepArgs& la, hipStream_t st)
+{
+    const size_t n = (size_t)rp * nt;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (p == kFP64) hipLaunchKernelGGL(k_lstep<PolF64>, grid, dim3(256), 0, st, (const C2<double>*)b1k, nt, n, la);
+    else if (p == kFP32) hipLaunchKernelGGL(k_lstep<PolF32>, grid, dim3(256), 0, st, (const C2<float>*)b1k, nt, n, la);
+    else hipLaunchKernelGGL(k_lstep<PolMix>, grid, dim3(256), 0, st, (const C2<float>*)b1k, nt, n, la);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int launch_reduce_max(int nr, const double* linemax, DevStatus* s, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, st, nr, linemax, s);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int launch_convert(Prec p, const double* src64, void* dst, size_t n, int to_device, hipStream_t st)
+{
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (p == kFP64) {
+        if (to_device) hipLaunchKernelGGL(k_convert<double>, grid, dim3(256), 0, st, src64, (C2<double>*)dst, n);
+        else hipLaunchKernelGGL(k_convert_back<double>, grid, dim3(256), 0, st, (const C2<double>*)dst, (double*)src64, n);
+    } else {
+        if (to_device) hipLaunchKernelGGL(k_convert<float>, grid, dim3(256), 0, st, src64, (C2<float>*)dst, n);
+        else hipLaunchKernelGGL(k_convert_back<float>, grid, dim3(256), 0, st, (const C2<float>*)dst, (double*)src64, n);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int launch_avg_spectrum(Prec p, int nr, int nt, const void* field, const double* wts, double* out, hipStream_t st)
+{
+    const dim3 grid((nt + 255) / 256);
+    if (p == kFP64) hipLaunchKernelGGL(k_avg_spectrum<double>, grid, dim3(256), 0, st, nr, nt, (const C2<double>*)field, wts, out);
+    else hipLaunchKernelGGL(k_avg_spectrum<float>, grid, dim3(256), 0, st, nr, nt, (const C2<float>*)field, wts, out);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+
+}  // namespace gs
