@@ -104,6 +104,20 @@ __device__ __forceinline__ void dft_small(C2<R>* v)
     }
 }
 
+// v[r] *= w^r for r = 1..RADIX-1.  The powers come from a product tree
+// (w^r = w^(r/2) w^(r - r/2), depth <= 4 for RADIX <= 16), so a pass needs
+// one twiddle load instead of RADIX-1 and the rounding error stays at a few ulp.
+template <int RADIX, class R>
+__device__ __forceinline__ void twiddle_pow(C2<R>* v, C2<R> w)
+{
+    C2<R> p[RADIX];
+    p[1] = w;
+#pragma unroll
+    for (int r = 2; r < RADIX; ++r) p[r] = cmul(p[r / 2], p[r - r / 2]);
+#pragma unroll
+    for (int r = 1; r < RADIX; ++r) v[r] = cmul(v[r], p[r]);
+}
+
 // ------------------------------------------------------------------ block scans
 template <class V> struct AffR { V a, b; };               // x -> a x + b
 template <class V> struct AffC { V ar, ai, br, bi; };     // complex x -> a x + b

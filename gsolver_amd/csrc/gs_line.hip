@@ -24,7 +24,10 @@
 #include "gs_dev.h"
 
 #ifndef GS_NL_E32
-#define GS_NL_E32 16   // samples per thread of the FP32/mixed line kernels for N_T >= 1024
+#define GS_NL_E32 8    // samples per thread of the FP32/mixed line kernels for N_T >= 1024
+#endif
+#ifndef GS_NL_WAVES
+#define GS_NL_WAVES 1  // minimum waves per SIMD the RK4 kernel is register-allocated for
 #endif
 
 namespace gs {
@@ -68,13 +71,9 @@ __device__ __forceinline__ void spass(const C2<R>* __restrict__ src, C2<R>* __re
             for (int r = 0; r < RADIX; ++r) v[r] = src[pidx(j + r * NB, E)];
             const int k = j & (Ns - 1);
             if (Ns > 1) {
-                const int step = k * (T / (Ns * RADIX));
-#pragma unroll
-                for (int r = 1; r < RADIX; ++r) {
-                    C2<R> w = tw[r * step];
-                    if (SIGN > 0) w.y = -w.y;
-                    v[r] = cmul(v[r], w);
-                }
+                C2<R> w = tw[k * (T / (Ns * RADIX))];
+                if (SIGN > 0) w.y = -w.y;
+                twiddle_pow<RADIX>(v, w);
             }
             dft_small<RADIX, SIGN>(v);
             const int d = (j - k) * RADIX + k;
@@ -105,8 +104,8 @@ struct PassChain<R, T, NT, E, 1, SIGN, P> {
 };
 
 // forward (SIGN -1) FFT of the chunk-layout signal fac[e]*x[e]; the spectrum
-// (strided layout) is combined into out according to MODE:
-//   0: out = F     1: out += M[omega] F (complex table)     2: out += -i K[omega] F (real table)
+// (strided layout) times a per-omega factor G is added to out, with G by MODE:
+//   0: G = 1     1: G = M[omega] (complex table)     2: G = -i K[omega] (real table)
 template <int MODE, class R, int T, int NT, int E, class F>
 __device__ __forceinline__ void fft_fwd_acc(const C2<R> (&x)[E], const F (&fac)[E], C2<R> (&out)[E], C2<R>* b0,
                                             C2<R>* b1, const C2<R>* tw, const void* table)
@@ -122,19 +121,20 @@ __device__ __forceinline__ void fft_fwd_acc(const C2<R> (&x)[E], const F (&fac)[
     C2<R> y[E];
 #pragma unroll
     for (int r = 0; r < E; ++r) y[r] = src[pidx(tid + r * NT, E)];
-#pragma unroll
-    for (int r = 1; r < E; ++r) y[r] = cmul(y[r], tw[r * tid]);
+    twiddle_pow<E>(y, tw[tid]);
     dft_small<E, -1>(y);
 #pragma unroll
     for (int r = 0; r < E; ++r) {
+        C2<R> g;
         if constexpr (MODE == 0) {
-            out[r] = y[r];
+            g = y[r];
         } else if constexpr (MODE == 1) {
-            out[r] = cadd(out[r], cmul(((const C2<R>*)table)[tid + r * NT], y[r]));
+            g = cmul(((const C2<R>*)table)[tid + r * NT], y[r]);
         } else {
             const R k = ((const R*)table)[tid + r * NT];
-            out[r] = cadd(out[r], C2<R>{k * y[r].y, -k * y[r].x});
+            g = C2<R>{k * y[r].y, -k * y[r].x};
         }
+        out[r] = cadd(out[r], g);
     }
     __syncthreads();   // src may be overwritten by the next transform
 }
@@ -176,19 +176,19 @@ struct PlasmaScratch {
 // at sample jE-1), carrying the derivative d out/d g_j; the affine block scan
 // of these linearised chunk maps gives the next guesses.  Once the guesses
 // stop moving, one more exact pass from the converged starts produces rho.
-template <class S, int E, int NT>
-__device__ void plasma_scan2(const S (&Wv)[E], const S (&cv)[E], S (&rho)[E], const NLArgs& a,
+template <class S, int E, int NT, class Wt, class O>
+__device__ bool plasma_scan2(const Wt (&Wv)[E], const Wt (&cv)[E], O (&rho)[E], const NLArgs& a,
                              const PlasmaScratch& ps, int* flags)
 {
     const int tid = threadIdx.x;
     bool any = false;
 #pragma unroll
-    for (int e = 0; e < E; ++e) any |= Wv[e] > (S)0;
+    for (int e = 0; e < E; ++e) any |= Wv[e] > (Wt)0;
     const bool line_any = __syncthreads_or(any);
     if (!line_any && a.rho_init == 0.0) {
 #pragma unroll
         for (int e = 0; e < E; ++e) rho[e] = 0;
-        return;
+        return false;
     }
     const S dt = (S)a.dt, N = (S)a.neutral, invN = (S)a.inv_neutral, invtau = (S)a.inv_recomb;
     ps.W[tid] = (double)Wv[E - 1];
@@ -210,10 +210,10 @@ __device__ void plasma_scan2(const S (&Wv)[E], const S (&cv)[E], S (&rho)[E], co
                 x = rho0;
                 D = 0;
             } else {
-                const S Wp = e == 0 ? Wm1 : ((tid == 0 && e == 1) ? (S)0 : Wv[e - 1]);
-                const S cp = e == 0 ? cm1 : ((tid == 0 && e == 1) ? (S)0 : cv[e - 1]);
+                const S Wp = e == 0 ? Wm1 : ((tid == 0 && e == 1) ? (S)0 : (S)Wv[e - 1]);
+                const S cp = e == 0 ? cm1 : ((tid == 0 && e == 1) ? (S)0 : (S)cv[e - 1]);
                 S d;
-                x = heun_step(x, Wp, cp, Wv[e], cv[e], dt, N, invN, invtau, &d);
+                x = heun_step(x, Wp, cp, (S)Wv[e], (S)cv[e], dt, N, invN, invtau, &d);
                 D *= d;
             }
         }
@@ -236,10 +236,10 @@ __device__ void plasma_scan2(const S (&Wv)[E], const S (&cv)[E], S (&rho)[E], co
                     if (j == 0 && e == 0) {
                         x = rho0;
                     } else {
-                        const S Wp = e == 0 ? Wm1 : ((j == 0 && e == 1) ? (S)0 : Wv[e - 1]);
-                        const S cp = e == 0 ? cm1 : ((j == 0 && e == 1) ? (S)0 : cv[e - 1]);
+                        const S Wp = e == 0 ? Wm1 : ((j == 0 && e == 1) ? (S)0 : (S)Wv[e - 1]);
+                        const S cp = e == 0 ? cm1 : ((j == 0 && e == 1) ? (S)0 : (S)cv[e - 1]);
                         S d;
-                        x = heun_step(x, Wp, cp, Wv[e], cv[e], dt, N, invN, invtau, &d);
+                        x = heun_step(x, Wp, cp, (S)Wv[e], (S)cv[e], dt, N, invN, invtau, &d);
                     }
                 }
                 ps.rho[j] = (double)x;
@@ -255,13 +255,14 @@ __device__ void plasma_scan2(const S (&Wv)[E], const S (&cv)[E], S (&rho)[E], co
         if (tid == 0 && e == 0) {
             x = rho0;
         } else {
-            const S Wp = e == 0 ? Wm1 : ((tid == 0 && e == 1) ? (S)0 : Wv[e - 1]);
-            const S cp = e == 0 ? cm1 : ((tid == 0 && e == 1) ? (S)0 : cv[e - 1]);
+            const S Wp = e == 0 ? Wm1 : ((tid == 0 && e == 1) ? (S)0 : (S)Wv[e - 1]);
+            const S cp = e == 0 ? cm1 : ((tid == 0 && e == 1) ? (S)0 : (S)cv[e - 1]);
             S d;
-            x = heun_step(x, Wp, cp, Wv[e], cv[e], dt, N, invN, invtau, &d);
+            x = heun_step(x, Wp, cp, (S)Wv[e], (S)cv[e], dt, N, invN, invtau, &d);
         }
-        rho[e] = x;
+        rho[e] = (O)x;
     }
+    return true;
 }
 
 // ------------------------------------------------------------------ NL RHS
@@ -282,9 +283,14 @@ __device__ __forceinline__ PlasmaScratch scratch_of(LineSmem<Pol, T, E>& sm)
     return PlasmaScratch{sm.pW, sm.pc, sm.prho, (void*)sm.scan, sm.red};
 }
 
-// calculate_NLresponse of one line: x spectral (strided layout) -> out spectral (strided)
+// calculate_NLresponse of one line: x spectral (strided layout) -> out spectral (strided).
+// out = FFT(loss) + PLASMA_FUNC FFT(rho E) - i KERR FFT(((1-fR) I + R*I) E)
+// (propagator.cu:629-783), accumulated in that order.  A term whose time signal
+// is zero on the whole line (no sample above the loss threshold, rho = 0 when
+// nothing ionizes and there are no ambient carriers) has a zero spectrum and
+// its FFT is skipped.
 template <class Pol, int T, int E>
-__device__ void nl_rhs(const C2<typename Pol::R> (&x)[E], C2<typename Pol::R> (&out)[E], LineSmem<Pol, T, E>& sm,
+__device__ __forceinline__ void nl_rhs(const C2<typename Pol::R> (&x)[E], C2<typename Pol::R> (&out)[E], LineSmem<Pol, T, E>& sm,
                        const NLArgs& a, int* flags)
 {
     using R = typename Pol::R;
@@ -298,27 +304,30 @@ __device__ void nl_rhs(const C2<typename Pol::R> (&x)[E], C2<typename Pol::R> (&
     C2<R> Ev[E];
     fft_inv<R, T, NT, E>(x, Ev, sm.b0, sm.b1, tw);
     R lossf[E];
-    S rho[E];
+    R rho[E];
+    bool anyloss = false, has_rho;
     {
-        S Ws[E], cs[E];
+        W Ws[E], cs[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             Ev[e] = cscale(Ev[e], (R)a.inv_nt);
             const W I = (W)Ev[e].x * (W)Ev[e].x + (W)Ev[e].y * (W)Ev[e].y;
             const W Wr = photoionization<W>(I, a.ionrate, flags);
-            Ws[e] = (S)Wr;
-            cs[e] = (S)((W)a.sigma / ((W)a.U + (W)a.pond * I) * I);
+            Ws[e] = Wr;
+            cs[e] = (W)a.sigma / ((W)a.U + (W)a.pond * I) * I;
             const int n = n0 + e;
             // loss[n] = -W (U + p I)/I E for 1 <= n <= T-2 above the ln-threshold (quirk, ionization.cpp:43)
             lossf[e] = (n >= 1 && n <= T - 2 && I > (W)29.933606208922594)
                            ? (R)(-(W)0.5 * Wr * ((W)a.U + (W)a.pond * I) / I) : (R)0;
+            anyloss |= lossf[e] != (R)0;
         }
-        plasma_scan2<S, E, NT>(Ws, cs, rho, a, scratch_of(sm), flags);
+        has_rho = plasma_scan2<S, E, NT>(Ws, cs, rho, a, scratch_of(sm), flags);
     }
 
-    // FFT(loss) + PLASMA_FUNC * FFT(rho E)
-    fft_fwd_acc<0, R, T, NT, E>(Ev, lossf, out, sm.b0, sm.b1, tw, nullptr);
-    fft_fwd_acc<1, R, T, NT, E>(Ev, rho, out, sm.b0, sm.b1, tw, a.pf);
+#pragma unroll
+    for (int r = 0; r < E; ++r) out[r] = C2<R>{0, 0};
+    if (__syncthreads_or(anyloss)) fft_fwd_acc<0, R, T, NT, E>(Ev, lossf, out, sm.b0, sm.b1, tw, nullptr);
+    if (has_rho) fft_fwd_acc<1, R, T, NT, E>(Ev, rho, out, sm.b0, sm.b1, tw, a.pf);
     if (!a.use_kerr) return;
 
     // Raman: s_n = z s_{n-1} + I_n; y = (1-fR) I + fR M Im(s (1 - z^T) + z^{n+1} s_{T-1})
@@ -326,24 +335,27 @@ __device__ void nl_rhs(const C2<typename Pol::R> (&x)[E], C2<typename Pol::R> (&
     if (a.use_raman) {
         const C2<R>* zp = (const C2<R>*)a.zpow;
         const C2<R> z = zp[1];
-        C2<R> sl[E];
-        C2<R> acc = {0, 0};
+        C2<R> loc = {0, 0};
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            acc = cmul(z, acc);
-            acc.x += Ev[e].x * Ev[e].x + Ev[e].y * Ev[e].y;
-            sl[e] = acc;
+            loc = cmul(z, loc);
+            loc.x += Ev[e].x * Ev[e].x + Ev[e].y * Ev[e].y;
         }
         const C2<R> zE = zp[E];
         AffC<R> tot;
-        const AffC<R> pre = block_scan_excl<NT>(AffC<R>{zE.x, zE.y, acc.x, acc.y}, (AffC<R>*)sm.scan, &tot);
-        const C2<R> sstart = {pre.br, pre.bi}, slast = {tot.br, tot.bi};
+        const AffC<R> pre = block_scan_excl<NT>(AffC<R>{zE.x, zE.y, loc.x, loc.y}, (AffC<R>*)sm.scan, &tot);
+        const C2<R> slast = {tot.br, tot.bi};
         const C2<R> omz = {(R)a.onemzT_re, (R)a.onemzT_im};
+        C2<R> sv = {pre.br, pre.bi};     // s at sample n0 - 1
+        C2<R> zq = zp[n0 + 1];           // z^{n+1} at n = n0
 #pragma unroll
         for (int e = 0; e < E; ++e) {
-            const C2<R> s = cadd(sl[e], cmul(zp[e + 1], sstart));
-            const C2<R> q = cadd(cmul(s, omz), cmul(zp[n0 + e + 1], slast));
-            yv[e] = (R)a.one_m_fr * (Ev[e].x * Ev[e].x + Ev[e].y * Ev[e].y) + (R)a.frM * q.y;
+            const R I = Ev[e].x * Ev[e].x + Ev[e].y * Ev[e].y;
+            sv = cmul(z, sv);
+            sv.x += I;
+            const C2<R> q = cadd(cmul(sv, omz), cmul(zq, slast));
+            yv[e] = (R)a.one_m_fr * I + (R)a.frM * q.y;
+            zq = cmul(zq, z);
         }
     } else {
 #pragma unroll
@@ -353,7 +365,7 @@ __device__ void nl_rhs(const C2<typename Pol::R> (&x)[E], C2<typename Pol::R> (&
 }
 
 template <class Pol, int T, int E>
-__global__ __launch_bounds__((Shape<Pol, T, E>::NT)) void k_nl_rk4(const C2<typename Pol::R>* __restrict__ field,
+__global__ __launch_bounds__((Shape<Pol, T, E>::NT), GS_NL_WAVES) void k_nl_rk4(const C2<typename Pol::R>* __restrict__ field,
                                                                     C2<typename Pol::R>* __restrict__ b1, double dz,
                                                                     NLArgs a, double* __restrict__ linemax)
 {
@@ -457,6 +469,8 @@ __global__ __launch_bounds__((FftShape<R, T>::NT)) void k_fft_lines(const C2<R>*
             x[e] = in[base + n];
             f[e] = (R)(scale * (filt ? filt[n] : 1.0));
         }
+#pragma unroll
+        for (int r = 0; r < E; ++r) y[r] = C2<R>{0, 0};
         fft_fwd_acc<0, R, T, NT, E>(x, f, y, b0, b1, tw, nullptr);
 #pragma unroll
         for (int r = 0; r < E; ++r) out[base + tid + r * NT] = y[r];
