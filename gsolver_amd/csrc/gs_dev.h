@@ -255,56 +255,53 @@ __device__ __forceinline__ S heun_step(S x, S Wp, S cp, S Wn, S cn, S dt, S N, S
 }
 
 // ------------------------------------------------------------------ linear step
-__device__ __forceinline__ double sqrtho_d(double x)
+// e^{i ph} for the linear-step phase.  MIXED reduces ph modulo 2 pi in FP64
+// (2 pi split in two doubles) and evaluates the phasor of the reduced angle
+// in FP32; FP64 and FP32 use their own precision throughout.
+template <class Pol>
+__device__ __forceinline__ void lstep_phasor(typename Pol::P ph, typename Pol::R* sn, typename Pol::R* cs)
 {
-    // sqrt(1+x)-1 as the 20-term Taylor sum of extmath.h:245-252, Horner form
-    const double c[20] = {0.5, -0.125, 0.0625, -0.0390625, 0.02734375, -0.0205078125, 0.01611328125,
-                          -0.013092041015625, 0.010910034179688, -0.009273529052734, 0.008008956909180,
-                          -0.007007837295532, 0.006199240684509, -0.005535036325455, 0.004981532692909,
-                          -0.004514514002949, 0.004116174532101, -0.003773159987759, 0.003475278936094,
-                          -0.003214633015887};
-    double r = c[19];
-#pragma unroll
-    for (int j = 18; j >= 0; --j) r = r * x + c[j];
-    return r * x;
-}
-__device__ __forceinline__ float sqrtho_f(float x)
-{
-    const float c[20] = {0.5f, -0.125f, 0.0625f, -0.0390625f, 0.02734375f, -0.0205078125f, 0.01611328125f,
-                         -0.013092041015625f, 0.010910034179688f, -0.009273529052734f, 0.008008956909180f,
-                         -0.007007837295532f, 0.006199240684509f, -0.005535036325455f, 0.004981532692909f,
-                         -0.004514514002949f, 0.004116174532101f, -0.003773159987759f, 0.003475278936094f,
-                         -0.003214633015887f};
-    float r = c[19];
-#pragma unroll
-    for (int j = 18; j >= 0; --j) r = r * x + c[j];
-    return r * x;
+    using R = typename Pol::R;
+    using P = typename Pol::P;
+    if constexpr (sizeof(R) == 8) {
+        sincos(ph, sn, cs);
+    } else if constexpr (sizeof(P) == 4) {
+        sincosf(ph, sn, cs);
+    } else {
+        const double q = rint(ph * 0.15915494309189535);
+        double r = fma(-q, 6.283185307179586, ph);
+        r = fma(-q, 2.4492935982947064e-16, r);
+        sincosf((float)r, sn, cs);
+    }
 }
 
+// one point of calculate_Lstep: B2 += dz B1k g; FIELD = B2 e^{i dz rate} amp; B2 = FIELD,
+// with amp = e^{dz Im k(omega)} of the point's column
 template <class Pol>
-__device__ __forceinline__ void lstep_point(int i, int w, size_t o, typename Pol::R yr, typename Pol::R yi, const LstepArgs& la)
+__device__ __forceinline__ void lstep_point(size_t o, typename Pol::R yr, typename Pol::R yi, typename Pol::R amp,
+                                            const LstepArgs& la)
 {
     using R = typename Pol::R;
     using P = typename Pol::P;
     C2<R>* fk = (C2<R>*)la.fk;
     C2<R>* b2 = (C2<R>*)la.b2;
-    if (i >= la.nr || !la.wok[w]) { fk[o] = C2<R>{0, 0}; return; }
-    const P kt2 = (P)la.kt2[i];
-    if (kt2 > (P)la.lim[w]) { fk[o] = C2<R>{0, 0}; return; }
-    const P k1 = (P)la.k1[w];
-    const P s = sizeof(P) == 8 ? (P)sqrtho_d((double)(-kt2 / k1 / k1)) : (P)sqrtho_f((float)(-kt2 / k1 / k1));
-    const P dz = (P)la.dz;
+    const R g = ((const R*)la.gfac)[o];
+    if (g < (R)0) { fk[o] = C2<R>{0, 0}; return; }
+    R sn, cs;
+    lstep_phasor<Pol>((P)la.dz * ((const P*)la.rate)[o], &sn, &cs);
     const C2<R> b = b2[o];
-    const P br = (P)b.x + dz * (P)yr / ((P)1 + s);
-    const P bi = (P)b.y + dz * (P)yi / ((P)1 + s);
-    const P ph = dz * (-(P)la.ho[w] - s * k1);
-    const P amp = exp(dz * (P)la.k2[w]);
-    P sn, cs;
-    if constexpr (sizeof(P) == 8) sincos(ph, &sn, &cs);
-    else sincosf(ph, &sn, &cs);
-    const C2<R> f = {(R)(amp * (br * cs - bi * sn)), (R)(amp * (br * sn + bi * cs))};
+    const R dg = (R)la.dz * g;
+    const R br = b.x + dg * yr, bi = b.y + dg * yi;
+    const C2<R> f = {amp * (br * cs - bi * sn), amp * (br * sn + bi * cs)};
     fk[o] = f;
     b2[o] = f;
+}
+
+template <class Pol>
+__device__ __forceinline__ typename Pol::R lstep_amp(int w, const LstepArgs& la)
+{
+    using P = typename Pol::P;
+    return (typename Pol::R)exp((P)la.dz * (P)la.k2[w]);
 }
 
 template <class Pol>
@@ -312,9 +309,8 @@ __global__ void k_lstep(const C2<typename Pol::R>* __restrict__ b1k, int nt, siz
 {
     const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= n) return;
-    const int i = (int)(o / nt), w = (int)(o % nt);
     const C2<typename Pol::R> y = b1k[o];
-    lstep_point<Pol>(i, w, o, y.x, y.y, la);
+    lstep_point<Pol>(o, y.x, y.y, lstep_amp<Pol>((int)(o % nt), la), la);
 }
 
 }  // namespace gs

@@ -46,18 +46,31 @@ struct NLArgs {
     int nl_e;                   // samples per thread of the RK4 kernel (0: default)
 };
 
+// Linear step (propagator.cu:570-625) from per-point set-up tables:
+//   rate[i][w] = -HO[w] - s k1[w],  g[i][w] = 1/(1+s),  s = sqrtHO(-kt2_i / k1_w^2)
+// with g < 0 marking points where FIELD is zeroed and B2 left alone (omega
+// outside [OMEGA_MIN, OMEGA_MAX], kt2 > MAX_KT2 Re(k^2), padding rows).
 struct LstepArgs {
-    int nr;                     // rows >= nr are padding: written as zero
     double dz;
-    const double* kt2;          // per row (padded)
-    const double* k1;           // Re k(omega)
     const double* k2;           // Im k(omega)
-    const double* ho;           // HO_DISPERSION
-    const double* lim;          // MAX_KT2 * Re(k*k)
-    const unsigned char* wok;   // OMEGA_MIN <= omega <= OMEGA_MAX
+    const void* rate;           // [rp][T], P precision of the policy
+    const void* gfac;           // [rp][T], field precision
     void* b2;                   // BIGBUFFER2 (C2<R>), updated in place
     void* fk;                   // FIELD in k-space (C2<R>), written
 };
+
+// sqrt(1+x) - 1 as the 20-term Taylor sum of extmath.h:245-252, Horner form
+constexpr double kSqrtHO[20] = {0.5, -0.125, 0.0625, -0.0390625, 0.02734375, -0.0205078125, 0.01611328125,
+                                -0.013092041015625, 0.010910034179688, -0.009273529052734, 0.008008956909180,
+                                -0.007007837295532, 0.006199240684509, -0.005535036325455, 0.004981532692909,
+                                -0.004514514002949, 0.004116174532101, -0.003773159987759, 0.003475278936094,
+                                -0.003214633015887};
+__host__ __device__ inline double sqrtho(double x)
+{
+    double r = kSqrtHO[19];
+    for (int j = 18; j >= 0; --j) r = r * x + kSqrtHO[j];
+    return r * x;
+}
 
 // ---- launchers (all asynchronous on `st`) ----
 // RK4 NL step over nr lines of field -> b1; per-line maxima into linemax[2*r]

@@ -97,7 +97,7 @@ struct gs_plan {
 
     DevBuf field, b1, b2, fk, tmp;       // [rp][T] in field precision
     DevBuf amat, amat_lo, at64;          // QDHT operators
-    DevBuf tw, tw64, zpow, pf, kerr, ionrate, kt2, k1, k2, ho, lim, wok, avgw;
+    DevBuf tw, tw64, zpow, pf, kerr, ionrate, k2, lrate, lgfac, avgw;
     DevBuf linemax, status, diag, et, rhot, scratch64a, scratch64b;
     std::vector<DevBuf> wfilt;
     gs::NLArgs nl{};
@@ -248,24 +248,37 @@ void build_plan(gs_plan* p)
         upload(p->kerr, d.data(), d.size() * 4);
     }
     upload(p->ionrate, s.ionrate_ln.data(), s.ionrate_ln.size() * 8);
-    std::vector<double> kt2(rp, 0.0), k1(nt), k2(nt), lim(nt);
-    std::vector<unsigned char> wok(nt);
-    for (int i = 0; i < nr; ++i) {
-        const double kt = 2 * M_PI * s.qx[i] * s.qV / s.XMAX;
-        kt2[i] = kt * kt;
+    {
+        // linear-step tables (propagator.cu:570-625): rate = -HO - s k1, g = 1/(1+s),
+        // s = sqrtHO(-kt2/k1^2); g = -1 where FIELD is zeroed (omega outside
+        // [OMEGA_MIN, OMEGA_MAX], kt2 > MAX_KT2 Re(k^2)) and on padding rows
+        std::vector<double> k2(nt), rate((size_t)rp * nt, 0.0), g((size_t)rp * nt, -1.0);
+        for (int w = 0; w < nt; ++w) k2[w] = s.wavenum[w].imag();
+        for (int i = 0; i < nr; ++i) {
+            const double kt = 2 * M_PI * s.qx[i] * s.qV / s.XMAX, kt2 = kt * kt;
+            for (int w = 0; w < nt; ++w) {
+                if (s.omega[w] < s.OMEGA_MIN || s.omega[w] > s.OMEGA_MAX) continue;
+                if (kt2 > 0.25 * (s.wavenum[w] * s.wavenum[w]).real()) continue;
+                const double k1 = s.wavenum[w].real();
+                const double sq = gs::sqrtho(-kt2 / k1 / k1);
+                rate[(size_t)i * nt + w] = -s.ho[w] - sq * k1;
+                g[(size_t)i * nt + w] = 1.0 / (1.0 + sq);
+            }
+        }
+        upload(p->k2, k2.data(), k2.size() * 8);
+        if (p->prec == gs::kFP32) {
+            std::vector<float> rf(rate.begin(), rate.end());
+            upload(p->lrate, rf.data(), rf.size() * 4);
+        } else {
+            upload(p->lrate, rate.data(), rate.size() * 8);
+        }
+        if (p->prec == gs::kFP64) {
+            upload(p->lgfac, g.data(), g.size() * 8);
+        } else {
+            std::vector<float> gf(g.begin(), g.end());
+            upload(p->lgfac, gf.data(), gf.size() * 4);
+        }
     }
-    for (int w = 0; w < nt; ++w) {
-        k1[w] = s.wavenum[w].real();
-        k2[w] = s.wavenum[w].imag();
-        lim[w] = 0.25 * (s.wavenum[w] * s.wavenum[w]).real();
-        wok[w] = !(s.omega[w] < s.OMEGA_MIN || s.omega[w] > s.OMEGA_MAX);
-    }
-    upload(p->kt2, kt2.data(), kt2.size() * 8);
-    upload(p->k1, k1.data(), k1.size() * 8);
-    upload(p->k2, k2.data(), k2.size() * 8);
-    upload(p->ho, s.ho.data(), s.ho.size() * 8);
-    upload(p->lim, lim.data(), lim.size() * 8);
-    upload(p->wok, wok.data(), wok.size());
     {
         // average_spectrum weights: x0 (x1 - x0), then x_r (x_r - x_{r-1})
         std::vector<double> wts(nr);
@@ -320,13 +333,9 @@ void build_plan(gs_plan* p)
     a.nl_e = 0;
     if (const char* e = std::getenv("GS_NL_E")) a.nl_e = std::atoi(e);
     gs::LstepArgs& l = p->ls;
-    l.nr = nr;
-    l.kt2 = p->kt2.as<double>();
-    l.k1 = p->k1.as<double>();
     l.k2 = p->k2.as<double>();
-    l.ho = p->ho.as<double>();
-    l.lim = p->lim.as<double>();
-    l.wok = p->wok.as<unsigned char>();
+    l.rate = p->lrate.p;
+    l.gfac = p->lgfac.p;
     l.b2 = p->b2.p;
     l.fk = p->fk.p;
 

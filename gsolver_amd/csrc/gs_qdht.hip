@@ -96,14 +96,15 @@ __global__ __launch_bounds__(256) void k_qdht_f32(const float* __restrict__ At, 
             __syncthreads();
         }
     }
+    const int col = n0 + wn + (lane & 31);
+    const float amp = LSTEP ? lstep_amp<PolF32>(col, la) : 1.f;
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int row = m0 + wm + rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const int col = n0 + wn + (lane & 31);
             const size_t o = (size_t)row * NC + col;
-            if (LSTEP) lstep_point<PolF32>(row, col, o, acc[rb][0][r], acc[rb][1][r], la);
+            if (LSTEP) lstep_point<PolF32>(o, acc[rb][0][r], acc[rb][1][r], amp, la);
             else Y[o] = C2<float>{acc[rb][0][r], acc[rb][1][r]};
         }
 }
@@ -209,14 +210,15 @@ __global__ __launch_bounds__(256) void k_qdht_bf16x3(const uint16_t* __restrict_
             __syncthreads();
         }
     }
+    const int col = n0 + wn + (lane & 31);
+    const float amp = LSTEP ? lstep_amp<PolMix>(col, la) : 1.f;
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int row = m0 + wm + rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const int col = n0 + wn + (lane & 31);
             const size_t o = (size_t)row * NC + col;
-            if (LSTEP) lstep_point<PolMix>(row, col, o, acc[rb][0][r], acc[rb][1][r], la);
+            if (LSTEP) lstep_point<PolMix>(o, acc[rb][0][r], acc[rb][1][r], amp, la);
             else Y[o] = C2<float>{acc[rb][0][r], acc[rb][1][r]};
         }
 }
@@ -281,15 +283,16 @@ __global__ __launch_bounds__(256) void k_qdht_f64(const double* __restrict__ At,
             __syncthreads();
         }
     }
+    const int col = n0 + wn + (lane & 15);
+    const double amp = LSTEP ? lstep_amp<PolF64>(col, la) : 1.0;
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             // v_mfma_f64_16x16x4_f64 result layout: row = lane/16 + 4 r, column = lane%16
             const int row = m0 + wm + rb * 16 + (lane >> 4) + 4 * r;
-            const int col = n0 + wn + (lane & 15);
             const size_t o = (size_t)row * NC + col;
-            if (LSTEP) lstep_point<PolF64>(row, col, o, acc[rb][0][r], acc[rb][1][r], la);
+            if (LSTEP) lstep_point<PolF64>(o, acc[rb][0][r], acc[rb][1][r], amp, la);
             else Y[o] = C2<double>{acc[rb][0][r], acc[rb][1][r]};
         }
 }
