@@ -82,11 +82,23 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 512 MiB device-to-device copy (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the z-march of one run is not sharded); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(2 ** 27, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(4):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": a.numel() * 4, "launches": 4}))
+        return
 
     import numpy as np
 

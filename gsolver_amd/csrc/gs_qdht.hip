@@ -13,6 +13,8 @@
 //   FP64   v_mfma_f64_16x16x4_f64, tile 64 x 32 complex, K step 16
 #include "gs_dev.h"
 
+#include <cstdlib>
+
 namespace gs {
 
 // ------------------------------------------------------------------ QDHT GEMMs
@@ -20,6 +22,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // blockIdx -> tile, grouping all row tiles of a column tile on one XCD so the
 // B columns are reused from that XCD's L2 (dispatch is round-robin over 8 XCDs)
@@ -109,11 +112,18 @@ __global__ __launch_bounds__(256) void k_qdht_f32(const float* __restrict__ At, 
         }
 }
 
-// BF16 3-pass: same tile, K step 32, operands k-contiguous in LDS
-__device__ __forceinline__ uint32_t bf16_rn(float f)
+// BF16 3-pass: same tile, K step 32, operands k-contiguous in LDS.
+// split2: two floats -> packed bf16 hi pair and lo pair (round to nearest
+// even, v_cvt_pk_bf16_f32), hi + lo = v to about 2^-17 relative
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split2(float v0, float v1, uint32_t* hi, uint32_t* lo)
 {
-    const uint32_t u = __float_as_uint(f);
-    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+    const f32x2_t v = {v0, v1};
+    const bf16x2_t h = __builtin_convertvector(v, bf16x2_t);
+    const bf16x2_t l = __builtin_convertvector(v - __builtin_convertvector(h, f32x2_t), bf16x2_t);
+    *hi = __builtin_bit_cast(uint32_t, h);
+    *lo = __builtin_bit_cast(uint32_t, l);
 }
 
 template <bool LSTEP>
@@ -160,10 +170,10 @@ __global__ __launch_bounds__(256) void k_qdht_bf16x3(const uint16_t* __restrict_
         x1b = *(const float4*)(p1 + 4);
     };
     auto split_store = [&](int c, int part, float v0, float v1) {
-        const uint32_t h0 = bf16_rn(v0), h1 = bf16_rn(v1);
-        const uint32_t l0 = bf16_rn(v0 - __uint_as_float(h0 << 16)), l1 = bf16_rn(v1 - __uint_as_float(h1 << 16));
-        *(uint32_t*)&sBh[part][c][2 * bkp] = h0 | (h1 << 16);
-        *(uint32_t*)&sBl[part][c][2 * bkp] = l0 | (l1 << 16);
+        uint32_t h, l;
+        split2(v0, v1, &h, &l);
+        *(uint32_t*)&sBh[part][c][2 * bkp] = h;
+        *(uint32_t*)&sBl[part][c][2 * bkp] = l;
     };
     auto sstore = [&]() {
         *(uint4*)&sAh[arow][ak] = rah0;
@@ -222,6 +232,197 @@ __global__ __launch_bounds__(256) void k_qdht_bf16x3(const uint16_t* __restrict_
             else Y[o] = C2<float>{acc[rb][0][r], acc[rb][1][r]};
         }
 }
+
+// BF16 3-pass, large tile: 128 rows x 128 complex columns per workgroup,
+// 4 waves of 64 x 64 complex, K step 32 (two MFMA k-slices of 16).
+// Software pipeline over double-buffered LDS, one barrier per K step:
+// while the MFMAs of step k read buffer k&1, the bf16 split of step k+1
+// (loaded from global during step k-1) is written to the other buffer and
+// the global loads of step k+2 are in flight.  B is loaded one column per
+// lane (16 k rows, coalesced across lanes) and stored k-contiguous.
+#ifndef GS_GEMM_SCHED
+#define GS_GEMM_SCHED 1
+#endif
+namespace t128 {
+constexpr int BM = 128, BNC = 128, BK = 32, LD = BK + 8;
+constexpr int PLANE = BM * LD;          // elements of one [128][LD] plane
+constexpr int BUF = 6 * PLANE;          // A hi, A lo, B hi re, B hi im, B lo re, B lo im
+constexpr size_t SMEM = (size_t)2 * BUF * sizeof(uint16_t);
+// global loads of one K step: A hi/lo (4 x 16 B), 16 complex B values of one column
+__device__ __forceinline__ void gload(const uint16_t* gAh, const uint16_t* gAl, const C2<float>* gX, int NC, int step,
+                                      int nk, u32x4 (&ga)[4], float (&xr)[16], float (&xi)[16])
+{
+    step = step < nk ? step : nk - 1;      // past the end: reload the last step (never used)
+    const size_t ka = (size_t)step * BK;
+    ga[0] = *(const u32x4*)(gAh + ka);
+    ga[1] = *(const u32x4*)(gAh + ka + 8);
+    ga[2] = *(const u32x4*)(gAl + ka);
+    ga[3] = *(const u32x4*)(gAl + ka + 8);
+    const C2<float>* xp = gX + ka * NC;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const C2<float> v = xp[(size_t)j * NC];
+        xr[j] = v.x;
+        xi[j] = v.y;
+    }
+}
+// split of one K step into the LDS buffer b
+__device__ __forceinline__ void sstore(uint16_t* b, int oA, int oB, const u32x4 (&ga)[4], const float (&xr)[16],
+                                       const float (&xi)[16])
+{
+    *(u32x4*)(b + oA) = ga[0];
+    *(u32x4*)(b + oA + 8) = ga[1];
+    *(u32x4*)(b + PLANE + oA) = ga[2];
+    *(u32x4*)(b + PLANE + oA + 8) = ga[3];
+    uint32_t hr[8], lr[8], hi[8], li[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        split2(xr[2 * j], xr[2 * j + 1], &hr[j], &lr[j]);
+        split2(xi[2 * j], xi[2 * j + 1], &hi[j], &li[j]);
+    }
+    *(u32x4*)(b + oB) = u32x4{hr[0], hr[1], hr[2], hr[3]};
+    *(u32x4*)(b + oB + 8) = u32x4{hr[4], hr[5], hr[6], hr[7]};
+    *(u32x4*)(b + oB + PLANE) = u32x4{hi[0], hi[1], hi[2], hi[3]};
+    *(u32x4*)(b + oB + PLANE + 8) = u32x4{hi[4], hi[5], hi[6], hi[7]};
+    *(u32x4*)(b + oB + 2 * PLANE) = u32x4{lr[0], lr[1], lr[2], lr[3]};
+    *(u32x4*)(b + oB + 2 * PLANE + 8) = u32x4{lr[4], lr[5], lr[6], lr[7]};
+    *(u32x4*)(b + oB + 3 * PLANE) = u32x4{li[0], li[1], li[2], li[3]};
+    *(u32x4*)(b + oB + 3 * PLANE + 8) = u32x4{li[4], li[5], li[6], li[7]};
+}
+}  // namespace t128
+
+template <bool LSTEP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_qdht_bf16x3_t128(const uint16_t* __restrict__ Ah, const uint16_t* __restrict__ Al,
+                                                          const C2<float>* __restrict__ X, C2<float>* __restrict__ Y,
+                                                          int K, int NC, LstepArgs la)
+{
+    using namespace t128;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    uint16_t* const sm = reinterpret_cast<uint16_t*>(smem_raw);
+
+    const int M = K;
+    int mi, ni;
+    tile_of(M / BM, NC / BNC, &mi, &ni);
+    const int m0 = mi * BM, n0 = ni * BNC;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
+    const int arow = tid >> 1, ak = (tid & 1) * 16;      // A loader: row, k half
+    const int bcol = tid & 127, bk = (tid >> 7) * 16;     // B loader: column, k half
+    const int cl = lane & 31, kq = 8 * (lane >> 5);
+
+    // per-thread LDS offsets within a buffer (elements)
+    const int oA = arow * LD + ak;                        // + PLANE for A lo
+    const int oB = 2 * PLANE + bcol * LD + bk;            // + PLANE per B plane
+    const int rA = (wm + cl) * LD + kq;                   // + rb*32*LD, + PLANE for lo, + kk
+    const int rB = 2 * PLANE + (wn + cl) * LD + kq;       // + plane*PLANE, + cb*32*LD, + kk
+
+    const uint16_t* gAh = Ah + (size_t)(m0 + arow) * K + ak;
+    const uint16_t* gAl = Al + (size_t)(m0 + arow) * K + ak;
+    const C2<float>* gX = X + (size_t)bk * NC + n0 + bcol;
+    const int nk = K / BK;
+
+    v16f acc[2][2][2];    // [rb][part][cb]
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[a][b][c][r] = 0.f;
+
+    auto mfma_step = [&](const uint16_t* b) {
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            v8bf ah[2], al[2], bh[2][2], bl[2][2];
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb) {
+                ah[rb] = *(const v8bf*)(b + rA + rb * 32 * LD + kk);
+                al[rb] = *(const v8bf*)(b + PLANE + rA + rb * 32 * LD + kk);
+            }
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+                for (int cb = 0; cb < 2; ++cb) {
+                    bh[pt][cb] = *(const v8bf*)(b + rB + pt * PLANE + cb * 32 * LD + kk);
+                    bl[pt][cb] = *(const v8bf*)(b + rB + (2 + pt) * PLANE + cb * 32 * LD + kk);
+                }
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+                    for (int cb = 0; cb < 2; ++cb)
+                        acc[rb][pt][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[rb], bh[pt][cb], acc[rb][pt][cb], 0, 0, 0);
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+                    for (int cb = 0; cb < 2; ++cb)
+                        acc[rb][pt][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[rb], bl[pt][cb], acc[rb][pt][cb], 0, 0, 0);
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+                    for (int cb = 0; cb < 2; ++cb)
+                        acc[rb][pt][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[rb], bh[pt][cb], acc[rb][pt][cb], 0, 0, 0);
+        }
+    };
+    // interleave the 48 MFMAs of a step with its LDS reads, the split VALU,
+    // the LDS writes and the global loads
+    auto schedule = [&]() {
+        if constexpr (GS_GEMM_SCHED == 0) return;
+#pragma unroll
+        for (int i = 0; i < 12; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // DS read
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);   // VALU
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);   // DS write
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x020, 2, 0);   // VMEM read
+        }
+    };
+
+    uint16_t* const buf0 = sm;
+    uint16_t* const buf1 = sm + BUF;
+    u32x4 ga[4], gna[4];
+    float xr[16], xi[16], nxr[16], nxi[16];
+    gload(gAh, gAl, gX, NC, 0, nk, ga, xr, xi);
+    sstore(buf0, oA, oB, ga, xr, xi);
+    gload(gAh, gAl, gX, NC, 1, nk, ga, xr, xi);
+    __syncthreads();
+    for (int k = 0; k < nk; k += 2) {
+        gload(gAh, gAl, gX, NC, k + 2, nk, gna, nxr, nxi);
+        mfma_step(buf0);
+        sstore(buf1, oA, oB, ga, xr, xi);
+        schedule();
+        __syncthreads();
+        gload(gAh, gAl, gX, NC, k + 3, nk, ga, xr, xi);
+        mfma_step(buf1);
+        sstore(buf0, oA, oB, gna, nxr, nxi);
+        schedule();
+        __syncthreads();
+    }
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+        const int col = n0 + wn + cb * 32 + (lane & 31);
+        const float amp = LSTEP ? lstep_amp<PolMix>(col, la) : 1.f;
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = m0 + wm + rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const size_t o = (size_t)row * NC + col;
+                if (LSTEP) lstep_point<PolMix>(o, acc[rb][0][cb][r], acc[rb][1][cb][r], amp, la);
+                else Y[o] = C2<float>{acc[rb][0][cb][r], acc[rb][1][cb][r]};
+            }
+    }
+}
+constexpr size_t kBf16T128Smem = t128::SMEM;
 
 // FP64: tile 64 rows x 32 complex columns, K step 16, 4 waves (2x2)
 template <bool LSTEP>
@@ -357,6 +558,12 @@ int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* amat_lo, c
         const dim3 grid((rp / 128) * (nt / 64));
         if (ls) hipLaunchKernelGGL(k_qdht_f32<true>, grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
         else hipLaunchKernelGGL(k_qdht_f32<false>, grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
+    } else if (!ls && nt % 128 == 0 && !std::getenv("GS_GEMM_T64")) {
+        if (rp % 128) return -1001;
+        const dim3 grid((rp / 128) * (nt / 128));
+        auto kf = k_qdht_bf16x3_t128<false>;
+        HIPCHK(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBf16T128Smem));
+        hipLaunchKernelGGL(kf, grid, dim3(256), kBf16T128Smem, st, (const uint16_t*)amat, (const uint16_t*)amat_lo, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
     } else {
         if (rp % 128 || nt % 64) return -1001;
         const dim3 grid((rp / 128) * (nt / 64));
