@@ -1,7 +1,7 @@
 // gs_host.h -- host-side set-up of a gsolver_solid_qdht run: config lookup,
 // grids, dispersion, coefficient tables, Keldysh rate table, QDHT plan and
 // the analytic initial field.  Everything here is FP64 and runs once per
-// plan; the per-step work runs on the GPU (gs_device.hip).
+// plan; the per-step work runs on the GPU (gs_line.hip, gs_qdht.hip).
 #pragma once
 
 #include <complex>

@@ -35,8 +35,9 @@ typedef struct gs_plan gs_plan;
 enum gs_precision {
     GS_FP64 = 0,   /* all kernels in double: the FP64 GPU run results are checked against */
     GS_FP32 = 1,   /* plain FP32: FP32 MFMA QDHT, FP32 FFT, scans and propagator */
-    GS_MIXED = 2   /* split-BF16 3-pass MFMA QDHT with FP32 accumulation; FP32 FFT;
-                      FP64 plasma scan, W(I), propagator phase and step-control norms */
+    GS_MIXED = 2   /* split-BF16 3-pass MFMA QDHT with FP32 accumulation; FP32 FFT,
+                      Kerr/Raman and W(I); FP64 plasma-density scan, propagator
+                      phase reduction, step-control norms and dump integrals */
 };
 
 enum gs_status {

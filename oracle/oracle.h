@@ -20,7 +20,7 @@
  *
  * Parity pins (see tests/test_oracle_*.py): bin/omega.bin, bin/wavenum.bin,
  * the 128-slab linear-regime dumps bin/fil_SiO2__{f,fa,mI,pm,s}.udf,
- * scipy.special golden vectors for Bessel J0/J1/zeros, Dawson and the
+ * mpmath golden vectors (tests/golden/make_special.py) for Bessel J0/J1/zeros, Dawson and the
  * complete elliptic integrals, brute-force DFT for the FFT, closed forms
  * for the Raman convolution, the plasma recurrence and the RK4 combine.
  * Parity unpinned (no reference-held vector exists): the Keldysh rate table
