@@ -61,6 +61,57 @@ def cpu_baseline(cfg, dz, nsteps, threads):
                       % (nsteps, threads, dt)}
 
 
+def read_udf(path):
+    """A .udf dump (dump_types.cpp:25-72): "UDF1.0\\0", 6 int32 header, float64 data."""
+    import numpy as np
+    raw = open(path, "rb").read()
+    fs, cplx, tN, xN, yN, zN = (int(v) for v in np.frombuffer(raw[7:31], dtype=np.int32))
+    data = np.frombuffer(raw[31:], dtype=np.complex128 if cplx else np.float64)
+    return data.reshape(zN, -1)
+
+
+def accuracy_check(gsa, workloads, work, z_max, planes):
+    """Adaptive march (the reference controller, MAX_TOLERANCE 1e-4) of the
+    production workload to z_max with `planes` dump planes after z = 0, in
+    every precision; fluence (flux dump) and peak plasma density (plasma_max
+    dump) at each plane and the field at the last plane, compared with the
+    FP64 GPU run."""
+    import numpy as np
+    keys = dict(N_Z=planes + 1, Z_MAX=z_max, DUMPS_N=2, DUMPTYPE0="flux", DUMP_FILENAME0="acc_f.udf",
+                DUMPTYPE1="plasma_max", DUMP_FILENAME1="acc_pm.udf")
+    d = os.path.join(work, "acc")
+    os.makedirs(d, exist_ok=True)
+    cfg = workloads.write_config(d, "c1", **keys)
+    out = {}
+    for prec in ("fp64", "mixed", "fp32"):
+        od = os.path.join(d, prec)
+        os.makedirs(od, exist_ok=True)
+        p = gsa.Plan(cfg, precision=prec)
+        t0 = time.perf_counter()
+        p.march(od)
+        wall = time.perf_counter() - t0
+        out[prec] = {"field": p.get(), "flux": read_udf(os.path.join(od, "acc_f.udf")),
+                     "pm": read_udf(os.path.join(od, "acc_pm.udf")), "steps": int(p.scalar("STEP_N")),
+                     "rejected": int(p.scalar("n_reject")), "wall_s": wall, "z": list(p.table("znet"))}
+        p.close()
+        print("[bench] accuracy march %s: %d steps (%d rejected) to z = %.3g m in %.1f s"
+              % (prec, out[prec]["steps"], out[prec]["rejected"], z_max, wall), file=sys.stderr, flush=True)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b)) if np.linalg.norm(b) > 0 else float(np.abs(a).max())
+    ref = out["fp64"]
+    res = {"z_planes_m": ref["z"][1:], "steps": {k: v["steps"] for k, v in out.items()},
+           "rejected": {k: v["rejected"] for k, v in out.items()}}
+    for prec in ("mixed", "fp32"):
+        o = out[prec]
+        res[prec] = {
+            "field_rel_l2_last_plane": rel(o["field"], ref["field"]),
+            "fluence_rel_l2": [rel(o["flux"][k], ref["flux"][k]) for k in range(1, planes + 1)],
+            "peak_plasma_rel": [abs(float(o["pm"][k].max()) - float(ref["pm"][k].max())) / float(ref["pm"][k].max())
+                                if ref["pm"][k].max() > 0 else 0.0 for k in range(1, planes + 1)],
+        }
+    res["peak_plasma_fp64"] = [float(ref["pm"][k].max()) for k in range(1, planes + 1)]
+    return res
+
+
 def roofline(kind, ms, R, T, prec):
     if kind == "qdht":
         flops = 4.0 * R * R * T                 # one transform: real R x R times 2T real columns
@@ -82,6 +133,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the adaptive-march accuracy check")
+    ap.add_argument("--accuracy-zmax", type=float, default=5e-5,
+                    help="distance of the accuracy-check march in m (two dump planes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -130,6 +184,10 @@ def main():
     dominant = "nl" if km["nl_rk4"] >= km["qdht_lstep"] + km["qdht_back"] else "qdht"
     roof = roofline("qdht" if dominant == "qdht" else "nl", qdht_ms if dominant == "qdht" else km["nl_rk4"], R, T, "mixed")
 
+    acc = None
+    if not args.no_accuracy and not args.no_fp64:
+        acc = accuracy_check(gsa, workloads, work, args.accuracy_zmax, 2)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -153,6 +211,7 @@ def main():
         "fp32_value": res["fp32"]["steps_per_s"],
         "mixed_over_fp32": m["steps_per_s"] / res["fp32"]["steps_per_s"],
         "field_rel_l2_vs_fp64": err or None,
+        "accuracy_march_vs_fp64": acc,
         "kernel_ms_per_step": {p: res[p]["kernel_ms"] for p in res},
         "roofline": roof,
         "roofline_qdht": roofline("qdht", qdht_ms, R, T, "mixed"),
