@@ -175,7 +175,8 @@ struct PlasmaScratch {
 // runs its E Heun steps exactly from a guessed start value g_j (the density
 // at sample jE-1), carrying the derivative d out/d g_j; the affine block scan
 // of these linearised chunk maps gives the next guesses.  Once the guesses
-// stop moving, one more exact pass from the converged starts produces rho.
+// stop moving (the start each thread used is within the tolerance of the
+// one the scan returns) the values of that pass are rho.
 template <class S, int E, int NT, class Wt, class O>
 __device__ bool plasma_scan2(const Wt (&Wv)[E], const Wt (&cv)[E], O (&rho)[E], const NLArgs& a,
                              const PlasmaScratch& ps, int* flags)
@@ -216,6 +217,7 @@ __device__ bool plasma_scan2(const Wt (&Wv)[E], const Wt (&cv)[E], O (&rho)[E], 
                 x = heun_step(x, Wp, cp, (S)Wv[e], (S)cv[e], dt, N, invN, invtau, &d);
                 D *= d;
             }
+            rho[e] = (O)x;
         }
         AffR<S> tot;
         const AffR<S> pre = block_scan_excl<NT>(AffR<S>{D, x - D * g}, sc, &tot);
@@ -246,21 +248,20 @@ __device__ bool plasma_scan2(const Wt (&Wv)[E], const Wt (&cv)[E], O (&rho)[E], 
             }
             __syncthreads();
         }
-        g = tid > 0 ? (S)ps.rho[tid - 1] : (S)0;
-    }
-    // exact pass from the converged chunk start
-    S x = g;
+        // exact pass from the chunk starts of the sequential sweep
+        S x = tid > 0 ? (S)ps.rho[tid - 1] : (S)0;
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        if (tid == 0 && e == 0) {
-            x = rho0;
-        } else {
-            const S Wp = e == 0 ? Wm1 : ((tid == 0 && e == 1) ? (S)0 : (S)Wv[e - 1]);
-            const S cp = e == 0 ? cm1 : ((tid == 0 && e == 1) ? (S)0 : (S)cv[e - 1]);
-            S d;
-            x = heun_step(x, Wp, cp, (S)Wv[e], (S)cv[e], dt, N, invN, invtau, &d);
+        for (int e = 0; e < E; ++e) {
+            if (tid == 0 && e == 0) {
+                x = rho0;
+            } else {
+                const S Wp = e == 0 ? Wm1 : ((tid == 0 && e == 1) ? (S)0 : (S)Wv[e - 1]);
+                const S cp = e == 0 ? cm1 : ((tid == 0 && e == 1) ? (S)0 : (S)cv[e - 1]);
+                S d;
+                x = heun_step(x, Wp, cp, (S)Wv[e], (S)cv[e], dt, N, invN, invtau, &d);
+            }
+            rho[e] = (O)x;
         }
-        rho[e] = (O)x;
     }
     return true;
 }
@@ -305,23 +306,32 @@ __device__ __forceinline__ void nl_rhs(const C2<typename Pol::R> (&x)[E], C2<typ
     fft_inv<R, T, NT, E>(x, Ev, sm.b0, sm.b1, tw);
     R lossf[E];
     R rho[E];
-    bool anyloss = false, has_rho;
+    bool anyloss = false, has_rho = false;
     {
-        W Ws[E], cs[E];
+        W Ws[E];
+        bool anyw = false;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             Ev[e] = cscale(Ev[e], (R)a.inv_nt);
             const W I = (W)Ev[e].x * (W)Ev[e].x + (W)Ev[e].y * (W)Ev[e].y;
-            const W Wr = photoionization<W>(I, a.ionrate, flags);
-            Ws[e] = Wr;
-            cs[e] = (W)a.sigma / ((W)a.U + (W)a.pond * I) * I;
-            const int n = n0 + e;
-            // loss[n] = -W (U + p I)/I E for 1 <= n <= T-2 above the ln-threshold (quirk, ionization.cpp:43)
-            lossf[e] = (n >= 1 && n <= T - 2 && I > (W)29.933606208922594)
-                           ? (R)(-(W)0.5 * Wr * ((W)a.U + (W)a.pond * I) / I) : (R)0;
-            anyloss |= lossf[e] != (R)0;
+            Ws[e] = photoionization<W>(I, a.ionrate, flags);
+            anyw |= Ws[e] > (W)0;
         }
-        has_rho = plasma_scan2<S, E, NT>(Ws, cs, rho, a, scratch_of(sm), flags);
+        // avalanche and loss need W > 0 somewhere (or ambient carriers)
+        if (__syncthreads_or(anyw) || a.rho_init != 0.0) {
+            W cs[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const W I = (W)Ev[e].x * (W)Ev[e].x + (W)Ev[e].y * (W)Ev[e].y;
+                cs[e] = (W)a.sigma / ((W)a.U + (W)a.pond * I) * I;
+                const int n = n0 + e;
+                // loss[n] = -W (U + p I)/I E for 1 <= n <= T-2 above the ln-threshold (quirk, ionization.cpp:43)
+                lossf[e] = (n >= 1 && n <= T - 2 && I > (W)29.933606208922594)
+                               ? (R)(-(W)0.5 * Ws[e] * ((W)a.U + (W)a.pond * I) / I) : (R)0;
+                anyloss |= lossf[e] != (R)0;
+            }
+            has_rho = plasma_scan2<S, E, NT>(Ws, cs, rho, a, scratch_of(sm), flags);
+        }
     }
 
 #pragma unroll
@@ -353,8 +363,8 @@ __device__ __forceinline__ void nl_rhs(const C2<typename Pol::R> (&x)[E], C2<typ
             const R I = Ev[e].x * Ev[e].x + Ev[e].y * Ev[e].y;
             sv = cmul(z, sv);
             sv.x += I;
-            const C2<R> q = cadd(cmul(sv, omz), cmul(zq, slast));
-            yv[e] = (R)a.one_m_fr * I + (R)a.frM * q.y;
+            const R qy = sv.x * omz.y + sv.y * omz.x + zq.x * slast.y + zq.y * slast.x;
+            yv[e] = (R)a.one_m_fr * I + (R)a.frM * qy;
             zq = cmul(zq, z);
         }
     } else {
