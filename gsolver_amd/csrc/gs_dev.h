@@ -255,9 +255,11 @@ __device__ __forceinline__ S heun_step(S x, S Wp, S cp, S Wn, S cn, S dt, S N, S
 }
 
 // ------------------------------------------------------------------ linear step
-// e^{i ph} for the linear-step phase.  MIXED reduces ph modulo 2 pi in FP64
-// (2 pi split in two doubles) and evaluates the phasor of the reduced angle
-// in FP32; FP64 and FP32 use their own precision throughout.
+// e^{i ph} for the linear-step phase.  MIXED reduces ph to r in [-pi/4, pi/4]
+// and a quadrant q in FP64 (pi/2 split in two doubles), then evaluates sin r
+// and cos r in FP32 from their Taylor series (terms to r^9 and r^10: the
+// first omitted terms are below 2e-9 on the interval) and rotates by q.
+// FP64 and FP32 use sincos / sincosf in their own precision.
 template <class Pol>
 __device__ __forceinline__ void lstep_phasor(typename Pol::P ph, typename Pol::R* sn, typename Pol::R* cs)
 {
@@ -268,10 +270,15 @@ __device__ __forceinline__ void lstep_phasor(typename Pol::P ph, typename Pol::R
     } else if constexpr (sizeof(P) == 4) {
         sincosf(ph, sn, cs);
     } else {
-        const double q = rint(ph * 0.15915494309189535);
-        double r = fma(-q, 6.283185307179586, ph);
-        r = fma(-q, 2.4492935982947064e-16, r);
-        sincosf((float)r, sn, cs);
+        const double q = rint(ph * 0.63661977236758134);            // 2/pi
+        double rd = fma(-q, 1.5707963267948966, ph);                 // pi/2, high part
+        rd = fma(-q, 6.123233995736766e-17, rd);                     // pi/2, low part
+        const float r = (float)rd, r2 = r * r;
+        const float s = r * (1.f + r2 * (-1.f / 6 + r2 * (1.f / 120 + r2 * (-1.f / 5040 + r2 * (1.f / 362880)))));
+        const float c = 1.f + r2 * (-0.5f + r2 * (1.f / 24 + r2 * (-1.f / 720 + r2 * (1.f / 40320 + r2 * (-1.f / 3628800)))));
+        const int iq = (int)(((long long)q) & 3);
+        *sn = iq == 0 ? s : (iq == 1 ? c : (iq == 2 ? -s : -c));
+        *cs = iq == 0 ? c : (iq == 1 ? -s : (iq == 2 ? -c : s));
     }
 }
 
@@ -295,6 +302,42 @@ __device__ __forceinline__ void lstep_point(size_t o, typename Pol::R yr, typena
     const C2<R> f = {amp * (br * cs - bi * sn), amp * (br * sn + bi * cs)};
     fk[o] = f;
     b2[o] = f;
+}
+
+// lstep_point over NB points with all table loads issued before any store
+// (the stores could alias later loads, so a per-point loop would serialise
+// one memory round trip per point)
+template <class Pol, int NB>
+__device__ __forceinline__ void lstep_batch(const size_t (&o)[NB], const typename Pol::R (&yr)[NB],
+                                            const typename Pol::R (&yi)[NB], typename Pol::R amp, const LstepArgs& la)
+{
+    using R = typename Pol::R;
+    using P = typename Pol::P;
+    C2<R>* fk = (C2<R>*)la.fk;
+    C2<R>* b2 = (C2<R>*)la.b2;
+    R g[NB];
+    P rt[NB];
+    C2<R> b[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        g[j] = ((const R*)la.gfac)[o[j]];
+        rt[j] = ((const P*)la.rate)[o[j]];
+        b[j] = b2[o[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (g[j] < (R)0) {
+            fk[o[j]] = C2<R>{0, 0};
+            continue;
+        }
+        R sn, cs;
+        lstep_phasor<Pol>((P)la.dz * rt[j], &sn, &cs);
+        const R dg = (R)la.dz * g[j];
+        const R br = b[j].x + dg * yr[j], bi = b[j].y + dg * yi[j];
+        const C2<R> f = {amp * (br * cs - bi * sn), amp * (br * sn + bi * cs)};
+        fk[o[j]] = f;
+        b2[o[j]] = f;
+    }
 }
 
 template <class Pol>

@@ -202,13 +202,16 @@ void build_plan(gs_plan* p)
         std::vector<float> f(At.begin(), At.end());
         upload(p->amat, f.data(), f.size() * 4);
     } else {
-        // split-BF16: hi = bf16(a), lo = bf16(a - hi), row-major A[m][k]
+        // split-BF16: hi = bf16(a), lo = bf16(a - hi), tiled [m/128][k/32][128][32]
+        // (one 8 KiB block per row tile and K step of the GEMM)
         std::vector<uint16_t> hi(A.size()), lo(A.size());
-        for (size_t i = 0; i < A.size(); ++i) {
-            const float a = (float)A[i];
-            hi[i] = bf16_rn_host(a);
-            lo[i] = bf16_rn_host(a - bf16_to_float(hi[i]));
-        }
+        for (int m = 0; m < rp; ++m)
+            for (int k = 0; k < rp; ++k) {
+                const float a = (float)A[(size_t)m * rp + k];
+                const size_t t = ((size_t)(m / 128) * (rp / 32) + k / 32) * 4096 + (m % 128) * 32 + k % 32;
+                hi[t] = bf16_rn_host(a);
+                lo[t] = bf16_rn_host(a - bf16_to_float(hi[t]));
+            }
         upload(p->amat, hi.data(), hi.size() * 2);
         upload(p->amat_lo, lo.data(), lo.size() * 2);
     }

@@ -9,7 +9,8 @@
 //  linear step in registers.
 //   FP32   v_mfma_f32_32x32x2_f32, tile 128 x 64 complex, K step 16
 //   MIXED  split-BF16 3-pass (hi*hi + hi*lo + lo*hi, FP32 accumulate),
-//          v_mfma_f32_32x32x16_bf16, tile 128 x 64 complex, K step 32
+//          v_mfma_f32_32x32x16_bf16, tile 128 x 128 complex, K step 32,
+//          software-pipelined over double-buffered LDS
 //   FP64   v_mfma_f64_16x16x4_f64, tile 64 x 32 complex, K step 16
 #include "gs_dev.h"
 
@@ -126,113 +127,6 @@ __device__ __forceinline__ void split2(float v0, float v1, uint32_t* hi, uint32_
     *lo = __builtin_bit_cast(uint32_t, l);
 }
 
-template <bool LSTEP>
-__global__ __launch_bounds__(256) void k_qdht_bf16x3(const uint16_t* __restrict__ Ah, const uint16_t* __restrict__ Al,
-                                                     const C2<float>* __restrict__ X, C2<float>* __restrict__ Y, int K,
-                                                     int NC, LstepArgs la)
-{
-    constexpr int BM = 128, BNC = 64, BK = 32, LD = BK + 8;
-    __shared__ __align__(16) uint16_t sAh[BM][LD];
-    __shared__ __align__(16) uint16_t sAl[BM][LD];
-    __shared__ __align__(16) uint16_t sBh[2][BNC][LD];
-    __shared__ __align__(16) uint16_t sBl[2][BNC][LD];
-    const int M = K;
-    int mi, ni;
-    tile_of(M / BM, NC / BNC, &mi, &ni);
-    const int m0 = mi * BM, n0 = ni * BNC;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int wm = (w >> 1) * 64, wn = (w & 1) * 32;
-    // A loader: row tid/2, k half (tid&1)*16; B loader: k pair tid/16, 4 columns (tid&15)*4
-    const int arow = tid >> 1, ak = (tid & 1) * 16;
-    const int bkp = tid >> 4, bc = (tid & 15) * 4;
-
-    v16f acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int c = 0; c < 2; ++c)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[a][c][r] = 0.f;
-
-    uint4 rah0, rah1, ral0, ral1;
-    float4 x0a, x0b, x1a, x1b;
-    auto gload = [&](int k0) {
-        const size_t ao = (size_t)(m0 + arow) * K + k0 + ak;
-        rah0 = *(const uint4*)(Ah + ao);
-        rah1 = *(const uint4*)(Ah + ao + 8);
-        ral0 = *(const uint4*)(Al + ao);
-        ral1 = *(const uint4*)(Al + ao + 8);
-        const float* p0 = (const float*)(X + (size_t)(k0 + 2 * bkp) * NC + n0 + bc);
-        const float* p1 = (const float*)(X + (size_t)(k0 + 2 * bkp + 1) * NC + n0 + bc);
-        x0a = *(const float4*)p0;
-        x0b = *(const float4*)(p0 + 4);
-        x1a = *(const float4*)p1;
-        x1b = *(const float4*)(p1 + 4);
-    };
-    auto split_store = [&](int c, int part, float v0, float v1) {
-        uint32_t h, l;
-        split2(v0, v1, &h, &l);
-        *(uint32_t*)&sBh[part][c][2 * bkp] = h;
-        *(uint32_t*)&sBl[part][c][2 * bkp] = l;
-    };
-    auto sstore = [&]() {
-        *(uint4*)&sAh[arow][ak] = rah0;
-        *(uint4*)&sAh[arow][ak + 8] = rah1;
-        *(uint4*)&sAl[arow][ak] = ral0;
-        *(uint4*)&sAl[arow][ak + 8] = ral1;
-        split_store(bc + 0, 0, x0a.x, x1a.x); split_store(bc + 0, 1, x0a.y, x1a.y);
-        split_store(bc + 1, 0, x0a.z, x1a.z); split_store(bc + 1, 1, x0a.w, x1a.w);
-        split_store(bc + 2, 0, x0b.x, x1b.x); split_store(bc + 2, 1, x0b.y, x1b.y);
-        split_store(bc + 3, 0, x0b.z, x1b.z); split_store(bc + 3, 1, x0b.w, x1b.w);
-    };
-    gload(0);
-    sstore();
-    __syncthreads();
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        const bool more = k0 + BK < K;
-        if (more) gload(k0 + BK);
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 16) {
-            const int ko = kk + 8 * (lane >> 5), cl = lane & 31;
-            v8bf ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-            for (int rb = 0; rb < 2; ++rb) {
-                ah[rb] = *(const v8bf*)&sAh[wm + rb * 32 + cl][ko];
-                al[rb] = *(const v8bf*)&sAl[wm + rb * 32 + cl][ko];
-            }
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                bh[c] = *(const v8bf*)&sBh[c][wn + cl][ko];
-                bl[c] = *(const v8bf*)&sBl[c][wn + cl][ko];
-            }
-#pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    acc[rb][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[rb], bh[c], acc[rb][c], 0, 0, 0);
-                    acc[rb][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[rb], bl[c], acc[rb][c], 0, 0, 0);
-                    acc[rb][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[rb], bh[c], acc[rb][c], 0, 0, 0);
-                }
-        }
-        __syncthreads();
-        if (more) {
-            sstore();
-            __syncthreads();
-        }
-    }
-    const int col = n0 + wn + (lane & 31);
-    const float amp = LSTEP ? lstep_amp<PolMix>(col, la) : 1.f;
-#pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int row = m0 + wm + rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const size_t o = (size_t)row * NC + col;
-            if (LSTEP) lstep_point<PolMix>(o, acc[rb][0][r], acc[rb][1][r], amp, la);
-            else Y[o] = C2<float>{acc[rb][0][r], acc[rb][1][r]};
-        }
-}
-
 // BF16 3-pass, large tile: 128 rows x 128 complex columns per workgroup,
 // 4 waves of 64 x 64 complex, K step 32 (two MFMA k-slices of 16).
 // Software pipeline over double-buffered LDS, one barrier per K step:
@@ -253,11 +147,11 @@ __device__ __forceinline__ void gload(const uint16_t* gAh, const uint16_t* gAl, 
                                       int nk, u32x4 (&ga)[4], float (&xr)[16], float (&xi)[16])
 {
     step = step < nk ? step : nk - 1;      // past the end: reload the last step (never used)
-    const size_t ka = (size_t)step * BK;
-    ga[0] = *(const u32x4*)(gAh + ka);
-    ga[1] = *(const u32x4*)(gAh + ka + 8);
-    ga[2] = *(const u32x4*)(gAl + ka);
-    ga[3] = *(const u32x4*)(gAl + ka + 8);
+    const size_t ka = (size_t)step * BK, kt = (size_t)step * BM * BK;
+    ga[0] = *(const u32x4*)(gAh + kt);
+    ga[1] = *(const u32x4*)(gAh + kt + 8);
+    ga[2] = *(const u32x4*)(gAl + kt);
+    ga[3] = *(const u32x4*)(gAl + kt + 8);
     const C2<float>* xp = gX + ka * NC;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
@@ -316,8 +210,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     const int rA = (wm + cl) * LD + kq;                   // + rb*32*LD, + PLANE for lo, + kk
     const int rB = 2 * PLANE + (wn + cl) * LD + kq;       // + plane*PLANE, + cb*32*LD, + kk
 
-    const uint16_t* gAh = Ah + (size_t)(m0 + arow) * K + ak;
-    const uint16_t* gAl = Al + (size_t)(m0 + arow) * K + ak;
+    // A is stored tiled, [m tile][k step][BM][BK], so a K step of a row tile is 8 KiB contiguous
+    const size_t atile = (size_t)mi * (K / BK) * (BM * BK) + arow * BK + ak;
+    const uint16_t* gAh = Ah + atile;
+    const uint16_t* gAl = Al + atile;
     const C2<float>* gX = X + (size_t)bk * NC + n0 + bcol;
     const int nk = K / BK;
 
@@ -331,59 +227,85 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
                 for (int r = 0; r < 16; ++r) acc[a][b][c][r] = 0.f;
 
-    auto mfma_step = [&](const uint16_t* b) {
+    // all MFMA operands of one K step (both k-slices) come out of LDS before
+    // the first MFMA, so the MFMA stream never waits on an LDS read
+    struct Ops {
+        v8bf ah[2][2], al[2][2], bh[2][2][2], bl[2][2][2];   // [kk][rb] / [kk][pt][cb]
+    };
+    auto read_ops = [&](const uint16_t* b, Ops& o) {
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 16) {
-            v8bf ah[2], al[2], bh[2][2], bl[2][2];
+        for (int q = 0; q < 2; ++q) {
+            const int kk = 16 * q;
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb) {
-                ah[rb] = *(const v8bf*)(b + rA + rb * 32 * LD + kk);
-                al[rb] = *(const v8bf*)(b + PLANE + rA + rb * 32 * LD + kk);
+                o.ah[q][rb] = *(const v8bf*)(b + rA + rb * 32 * LD + kk);
+                o.al[q][rb] = *(const v8bf*)(b + PLANE + rA + rb * 32 * LD + kk);
             }
 #pragma unroll
             for (int pt = 0; pt < 2; ++pt)
 #pragma unroll
                 for (int cb = 0; cb < 2; ++cb) {
-                    bh[pt][cb] = *(const v8bf*)(b + rB + pt * PLANE + cb * 32 * LD + kk);
-                    bl[pt][cb] = *(const v8bf*)(b + rB + (2 + pt) * PLANE + cb * 32 * LD + kk);
+                    o.bh[q][pt][cb] = *(const v8bf*)(b + rB + pt * PLANE + cb * 32 * LD + kk);
+                    o.bl[q][pt][cb] = *(const v8bf*)(b + rB + (2 + pt) * PLANE + cb * 32 * LD + kk);
                 }
-#pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-                for (int pt = 0; pt < 2; ++pt)
-#pragma unroll
-                    for (int cb = 0; cb < 2; ++cb)
-                        acc[rb][pt][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[rb], bh[pt][cb], acc[rb][pt][cb], 0, 0, 0);
-#pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-                for (int pt = 0; pt < 2; ++pt)
-#pragma unroll
-                    for (int cb = 0; cb < 2; ++cb)
-                        acc[rb][pt][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[rb], bl[pt][cb], acc[rb][pt][cb], 0, 0, 0);
-#pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-                for (int pt = 0; pt < 2; ++pt)
-#pragma unroll
-                    for (int cb = 0; cb < 2; ++cb)
-                        acc[rb][pt][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[rb], bh[pt][cb], acc[rb][pt][cb], 0, 0, 0);
         }
     };
-    // interleave the 48 MFMAs of a step with its LDS reads, the split VALU,
-    // the LDS writes and the global loads
+    auto mfma_ops = [&](const Ops& o) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+                    for (int cb = 0; cb < 2; ++cb)
+                        acc[rb][pt][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(o.al[q][rb], o.bh[q][pt][cb], acc[rb][pt][cb], 0, 0, 0);
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+                    for (int cb = 0; cb < 2; ++cb)
+                        acc[rb][pt][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(o.ah[q][rb], o.bl[q][pt][cb], acc[rb][pt][cb], 0, 0, 0);
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+                    for (int cb = 0; cb < 2; ++cb)
+                        acc[rb][pt][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(o.ah[q][rb], o.bh[q][pt][cb], acc[rb][pt][cb], 0, 0, 0);
+        }
+    };
+    // interleave the 48 MFMAs of a step with the split VALU, the LDS writes
+    // and the global loads of the other stages
     auto schedule = [&]() {
         if constexpr (GS_GEMM_SCHED == 0) return;
 #pragma unroll
-        for (int i = 0; i < 12; ++i) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
-            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // DS read
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);   // VALU
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);   // DS write
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x020, 2, 0);   // VMEM read
+        for (int i = 0; i < 16; ++i) {
+            if constexpr (GS_GEMM_SCHED == 2) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);   // VALU
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);   // VMEM read
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+                __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);   // DS write
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+            } else if constexpr (GS_GEMM_SCHED == 3) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x020, 2, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+            } else {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);   // VALU
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);   // DS write
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x020, 2, 0);   // VMEM read
+            }
         }
     };
 
@@ -395,14 +317,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     sstore(buf0, oA, oB, ga, xr, xi);
     gload(gAh, gAl, gX, NC, 1, nk, ga, xr, xi);
     __syncthreads();
+    Ops ops;
     for (int k = 0; k < nk; k += 2) {
+        read_ops(buf0, ops);
+        __builtin_amdgcn_sched_barrier(0);
         gload(gAh, gAl, gX, NC, k + 2, nk, gna, nxr, nxi);
-        mfma_step(buf0);
+        mfma_ops(ops);
         sstore(buf1, oA, oB, ga, xr, xi);
         schedule();
         __syncthreads();
+        read_ops(buf1, ops);
+        __builtin_amdgcn_sched_barrier(0);
         gload(gAh, gAl, gX, NC, k + 3, nk, ga, xr, xi);
-        mfma_step(buf1);
+        mfma_ops(ops);
         sstore(buf0, oA, oB, gna, nxr, nxi);
         schedule();
         __syncthreads();
@@ -414,11 +341,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = m0 + wm + rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                const size_t o = (size_t)row * NC + col;
-                if (LSTEP) lstep_point<PolMix>(o, acc[rb][0][cb][r], acc[rb][1][cb][r], amp, la);
-                else Y[o] = C2<float>{acc[rb][0][cb][r], acc[rb][1][cb][r]};
+            for (int h = 0; h < 2; ++h) {
+                size_t o[8];
+                float yr[8], yi[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int r = 8 * h + j;
+                    const int row = m0 + wm + rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    o[j] = (size_t)row * NC + col;
+                    yr[j] = acc[rb][0][cb][r];
+                    yi[j] = acc[rb][1][cb][r];
+                }
+                if constexpr (LSTEP) {
+                    lstep_batch<PolMix, 8>(o, yr, yi, amp, la);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) Y[o[j]] = C2<float>{yr[j], yi[j]};
+                }
             }
     }
 }
@@ -558,17 +497,15 @@ int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* amat_lo, c
         const dim3 grid((rp / 128) * (nt / 64));
         if (ls) hipLaunchKernelGGL(k_qdht_f32<true>, grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
         else hipLaunchKernelGGL(k_qdht_f32<false>, grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
-    } else if (!ls && nt % 128 == 0 && !std::getenv("GS_GEMM_T64")) {
-        if (rp % 128) return -1001;
-        const dim3 grid((rp / 128) * (nt / 128));
-        auto kf = k_qdht_bf16x3_t128<false>;
-        HIPCHK(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBf16T128Smem));
-        hipLaunchKernelGGL(kf, grid, dim3(256), kBf16T128Smem, st, (const uint16_t*)amat, (const uint16_t*)amat_lo, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
     } else {
-        if (rp % 128 || nt % 64) return -1001;
-        const dim3 grid((rp / 128) * (nt / 64));
-        if (ls) hipLaunchKernelGGL(k_qdht_bf16x3<true>, grid, dim3(256), 0, st, (const uint16_t*)amat, (const uint16_t*)amat_lo, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
-        else hipLaunchKernelGGL(k_qdht_bf16x3<false>, grid, dim3(256), 0, st, (const uint16_t*)amat, (const uint16_t*)amat_lo, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
+        if (rp % 128 || nt % 128) return -1001;
+        const dim3 grid((rp / 128) * (nt / 128));
+        auto kt = k_qdht_bf16x3_t128<true>;
+        auto kf = k_qdht_bf16x3_t128<false>;
+        HIPCHK(hipFuncSetAttribute((const void*)kt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBf16T128Smem));
+        HIPCHK(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBf16T128Smem));
+        if (ls) hipLaunchKernelGGL(kt, grid, dim3(256), kBf16T128Smem, st, (const uint16_t*)amat, (const uint16_t*)amat_lo, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
+        else hipLaunchKernelGGL(kf, grid, dim3(256), kBf16T128Smem, st, (const uint16_t*)amat, (const uint16_t*)amat_lo, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
     }
     HIPCHK(hipGetLastError());
     return 0;
