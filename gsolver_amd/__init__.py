@@ -58,6 +58,7 @@ def lib():
         L.gs_line_diagnostics.argtypes = [vp, D, D, D]
         L.gs_fft_lines.argtypes = [vp, D, D, ip, ip]
         L.gs_last_timings.argtypes = [vp, D]
+        L.gs_fixed_steps.argtypes = [vp, dp, ip, D]
         _lib = L
     return _lib
 
@@ -177,6 +178,12 @@ class Plan:
         a, b = C.c_double(), C.c_double()
         _check(lib().gs_fixed_step(self.h, dz, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def fixed_steps(self, dz, n):
+        """n fixed-dz steps with one synchronisation; returns the mean kernel times per step."""
+        t = np.zeros(4)
+        _check(lib().gs_fixed_steps(self.h, dz, int(n), _d(t)))
+        return dict(zip(("nl_rk4", "qdht_lstep", "qdht_back", "reduce"), t))
 
     def march(self, outdir, max_steps=0):
         cwd = os.getcwd()

@@ -876,6 +876,52 @@ int gs_fft_lines(gs_plan* p, const double* in, double* out, int n_lines, int sig
     });
 }
 
+int gs_fixed_steps(gs_plan* p, double dz, int n, double* ms4)
+{
+    return guarded([&]() {
+        if (!p) return fail(GS_E_ARG, "null plan");
+        if (n <= 0) return fail(GS_E_ARG, "gs_fixed_steps: n must be positive");
+        std::vector<hipEvent_t> ev((size_t)5 * n, nullptr);
+        struct EvGuard {
+            std::vector<hipEvent_t>& v;
+            ~EvGuard() { for (auto e : v) if (e) (void)hipEventDestroy(e); }
+        } guard{ev};
+        for (auto& e : ev) HCK(hipEventCreate(&e));
+        const void* A = p->prec == gs::kFP64 ? p->at64.p : p->amat.p;
+        p->ls.dz = dz;
+        for (int k = 0; k < n; ++k) {
+            hipEvent_t* e = &ev[(size_t)5 * k];
+            HCK(hipEventRecord(e[0], p->st));
+            LCK(gs::launch_nl_rk4(p->prec, p->s.NT, p->field.p, p->b1.p, dz, p->nl, p->linemax.as<double>(), p->st));
+            HCK(hipEventRecord(e[1], p->st));
+            LCK(gs::launch_reduce_max(p->s.NR, p->linemax.as<double>(), p->status.as<gs::DevStatus>(), p->st));
+            HCK(hipEventRecord(e[2], p->st));
+            LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->amat_lo.p, p->b1.p, nullptr, &p->ls, p->st));
+            HCK(hipEventRecord(e[3], p->st));
+            LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->amat_lo.p, p->fk.p, p->field.p, nullptr, p->st));
+            HCK(hipEventRecord(e[4], p->st));
+        }
+        check_status(p, nullptr);     // synchronises; NaN / blast flags of any of the steps
+        double t[4] = {0, 0, 0, 0};
+        for (int k = 0; k < n; ++k) {
+            const hipEvent_t* e = &ev[(size_t)5 * k];
+            float a = 0, b = 0, c = 0, d = 0;
+            (void)hipEventElapsedTime(&a, e[0], e[1]);
+            (void)hipEventElapsedTime(&d, e[1], e[2]);
+            (void)hipEventElapsedTime(&b, e[2], e[3]);
+            (void)hipEventElapsedTime(&c, e[3], e[4]);
+            t[0] += a; t[1] += b; t[2] += c; t[3] += d;
+        }
+        for (int i = 0; i < 4; ++i) p->timings[i] = t[i] / n;
+        if (ms4)
+            for (int i = 0; i < 4; ++i) ms4[i] = p->timings[i];
+        for (int k = 0; k < n; ++k) p->CURRENT_Z += dz;
+        p->ZSTEP = dz;
+        p->STEP_N += n;
+        return 0;
+    });
+}
+
 int gs_last_timings(gs_plan* p, double* ms4)
 {
     if (!p || !ms4) return fail(GS_E_ARG, "null argument");

@@ -131,6 +131,12 @@ int gs_line_diagnostics(gs_plan* p, double* flux, double* maxI, double* rhomax);
  * unnormalised (the FFTW plans of solver_main.cpp:339-341). */
 int gs_fft_lines(gs_plan* p, const double* in, double* out, int n_lines, int sign);
 
+/* n steps of gs_fixed_step enqueued back to back on the plan's stream with one
+ * synchronisation at the end (the fixed-dz throughput mode of the benchmark;
+ * the reference has no counterpart).  ms4 (may be NULL) receives the mean
+ * device time per step of the four kernel groups, as gs_last_timings. */
+int gs_fixed_steps(gs_plan* p, double dz, int n, double* ms4);
+
 /* Device time of the kernels of the last gs_fixed_step / gs_step, in ms:
  * [0] NL RK4 kernel, [1] QDHT + linear step, [2] QDHT back, [3] reductions. */
 int gs_last_timings(gs_plan* p, double* ms4);
