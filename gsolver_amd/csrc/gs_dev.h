@@ -139,6 +139,48 @@ template <class V> __device__ __forceinline__ AffC<V> shfl_up(AffC<V> v, int d)
 template <class V> __device__ __forceinline__ AffR<V> aff_id(AffR<V>*) { return {(V)1, (V)0}; }
 template <class V> __device__ __forceinline__ AffC<V> aff_id(AffC<V>*) { return {(V)1, (V)0, (V)0, (V)0}; }
 
+// DPP lane moves (VALU, no LDS traffic): lanes whose source lane does not
+// exist, or whose row is masked off, receive `id` instead.
+//   row_shr:n 0x110+n (within rows of 16), row_bcast:15 0x142 (lane 15 of a
+//   row to the next row), row_bcast:31 0x143 (lane 31 to rows 2 and 3)
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_f(float id, float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, id), __builtin_bit_cast(int, v),
+                                                                  CTRL, ROWMASK, 0xf, false));
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f(double id, double v)
+{
+    const long long i = __builtin_bit_cast(long long, id), x = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp((int)i, (int)x, CTRL, ROWMASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(i >> 32), (int)(x >> 32), CTRL, ROWMASK, 0xf, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+template <int CTRL, int ROWMASK, class V>
+__device__ __forceinline__ AffR<V> dpp_aff(AffR<V> v)
+{
+    return {dpp_f<CTRL, ROWMASK>((V)1, v.a), dpp_f<CTRL, ROWMASK>((V)0, v.b)};
+}
+template <int CTRL, int ROWMASK, class V>
+__device__ __forceinline__ AffC<V> dpp_aff(AffC<V> v)
+{
+    return {dpp_f<CTRL, ROWMASK>((V)1, v.ar), dpp_f<CTRL, ROWMASK>((V)0, v.ai), dpp_f<CTRL, ROWMASK>((V)0, v.br),
+            dpp_f<CTRL, ROWMASK>((V)0, v.bi)};
+}
+// inclusive wave scan (lane i gets lanes 0..i composed, earlier lanes first)
+template <class A>
+__device__ __forceinline__ A wave_scan_incl(A v)
+{
+    v = compose(dpp_aff<0x111, 0xf>(v), v);
+    v = compose(dpp_aff<0x112, 0xf>(v), v);
+    v = compose(dpp_aff<0x114, 0xf>(v), v);
+    v = compose(dpp_aff<0x118, 0xf>(v), v);
+    v = compose(dpp_aff<0x142, 0xa>(v), v);
+    v = compose(dpp_aff<0x143, 0xc>(v), v);
+    return v;
+}
+
 // Exclusive block scan of affine maps (composition, earlier threads first).
 // lds must hold NW+1 entries.  Returns the prefix; *total = all threads composed.
 template <int NT, class A>
@@ -147,12 +189,7 @@ __device__ A block_scan_excl(A v, A* lds, A* total)
     constexpr int NW = NT / 64;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const A id = aff_id((A*)nullptr);
-    A inc = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        A o = shfl_up(inc, d);
-        if (lane >= d) inc = compose(o, inc);
-    }
+    const A inc = wave_scan_incl(v);
     if (lane == 63) lds[wid] = inc;
     __syncthreads();
     if (threadIdx.x == 0) {

@@ -374,6 +374,15 @@ __device__ __forceinline__ void nl_rhs(const C2<typename Pol::R> (&x)[E], C2<typ
     fft_fwd_acc<2, R, T, NT, E>(Ev, yv, out, sm.b0, sm.b1, tw, a.kerr);
 }
 
+// FP32 field lines keep the RK4 input row A and the k1 + 2k2 + 2k3 + k4
+// accumulator in LDS (2 x 32 KiB at N_T = 4096, after LineSmem); FP64 lines
+// do not fit and use the global rows.
+template <class Pol, int T>
+struct RK4Rows {
+    static constexpr bool in_lds = sizeof(typename Pol::R) == 4;
+    static constexpr size_t bytes = in_lds ? 2 * (size_t)T * sizeof(C2<typename Pol::R>) : 0;
+};
+
 template <class Pol, int T, int E>
 __global__ __launch_bounds__((Shape<Pol, T, E>::NT), GS_NL_WAVES) void k_nl_rk4(const C2<typename Pol::R>* __restrict__ field,
                                                                     C2<typename Pol::R>* __restrict__ b1, double dz,
@@ -381,16 +390,21 @@ __global__ __launch_bounds__((Shape<Pol, T, E>::NT), GS_NL_WAVES) void k_nl_rk4(
 {
     using R = typename Pol::R;
     constexpr int NT = Shape<Pol, T, E>::NT;
+    constexpr bool ROWS_LDS = RK4Rows<Pol, T>::in_lds;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     LineSmem<Pol, T, E>& sm = *reinterpret_cast<LineSmem<Pol, T, E>*>(smem_raw);
     const int line = blockIdx.x, tid = threadIdx.x;
-    const C2<R>* A = field + (size_t)line * T;
-    C2<R>* acc = b1 + (size_t)line * T;    // k1 + 2 k2 + 2 k3 + k4 accumulates in the output row
+    const C2<R>* Ag = field + (size_t)line * T;
+    C2<R>* b1row = b1 + (size_t)line * T;
+    // A and the accumulator: LDS rows (FP32) or the global rows (FP64, the output row accumulates)
+    C2<R>* Arow = ROWS_LDS ? reinterpret_cast<C2<R>*>(smem_raw + sizeof(LineSmem<Pol, T, E>)) : const_cast<C2<R>*>(Ag);
+    C2<R>* acc = ROWS_LDS ? Arow + T : b1row;
     int flags = 0;
     double mA = 0, mN = 0;
 #pragma unroll
     for (int r = 0; r < E; ++r) {
-        const C2<R> av = A[tid + r * NT];
+        const C2<R> av = Ag[tid + r * NT];
+        if constexpr (ROWS_LDS) Arow[tid + r * NT] = av;
         mA = fmax(mA, (double)av.x * av.x + (double)av.y * av.y);
     }
     C2<R> k[E];
@@ -398,7 +412,8 @@ __global__ __launch_bounds__((Shape<Pol, T, E>::NT), GS_NL_WAVES) void k_nl_rk4(
     for (int r = 0; r < E; ++r) k[r] = C2<R>{0, 0};
     const R rdz = (R)dz;
     // classical RK4 without linear propagation between stages: stage inputs
-    // A, A + dz/2 k1, A + dz/2 k2, A + dz k3; B1 = (k1 + 2 k2 + 2 k3 + k4)/6
+    // A, A + dz/2 k1, A + dz/2 k2, A + dz k3; B1 = (k1 + 2 k2 + 2 k3 + k4)/6.
+    // Each thread reads and writes only its own entries of A / acc.
 #pragma unroll 1
     for (int s = 0; s < 4; ++s) {
         const R c = s == 0 ? (R)0 : (s == 3 ? rdz : rdz * (R)0.5);
@@ -406,7 +421,7 @@ __global__ __launch_bounds__((Shape<Pol, T, E>::NT), GS_NL_WAVES) void k_nl_rk4(
         const R wold = s == 0 ? (R)0 : (R)1;
         C2<R> xin[E];
 #pragma unroll
-        for (int r = 0; r < E; ++r) xin[r] = cadd(A[tid + r * NT], cscale(k[r], c));
+        for (int r = 0; r < E; ++r) xin[r] = cadd(Arow[tid + r * NT], cscale(k[r], c));
         nl_rhs<Pol, T, E>(xin, k, sm, a, &flags);
 #pragma unroll
         for (int r = 0; r < E; ++r) {
@@ -420,7 +435,7 @@ __global__ __launch_bounds__((Shape<Pol, T, E>::NT), GS_NL_WAVES) void k_nl_rk4(
         const int o = tid + r * NT;
         const C2<R> t = acc[o];
         const C2<R> v = {t.x / (R)6, t.y / (R)6};
-        acc[o] = v;
+        b1row[o] = v;
         mN = fmax(mN, (double)v.x * v.x + (double)v.y * v.y);
     }
     block_max2<NT>(mA, mN, sm.red);
@@ -574,7 +589,7 @@ template <class Pol, int T, int E>
 static int nl_rk4_TE(const void* field, void* b1, double dz, const NLArgs& a, double* linemax, hipStream_t st)
 {
     using R = typename Pol::R;
-    const size_t sm = sizeof(LineSmem<Pol, T, E>);
+    const size_t sm = sizeof(LineSmem<Pol, T, E>) + RK4Rows<Pol, T>::bytes;
     auto kf = k_nl_rk4<Pol, T, E>;
     HIPCHK(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
     hipLaunchKernelGGL(kf, dim3(a.nr), dim3(Shape<Pol, T, E>::NT), sm, st, (const C2<R>*)field, (C2<R>*)b1, dz, a, linemax);
