@@ -183,30 +183,47 @@ __device__ __forceinline__ A wave_scan_incl(A v)
 
 // Exclusive block scan of affine maps (composition, earlier threads first).
 // lds must hold NW+1 entries.  Returns the prefix; *total = all threads composed.
+// value of lane `l` (wave-uniform) broadcast to the whole wave
+__device__ __forceinline__ float lane_bcast(float v, int l)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double lane_bcast(double v, int l)
+{
+    const long long x = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)x, l), hi = __builtin_amdgcn_readlane((int)(x >> 32), l);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+template <class V> __device__ __forceinline__ AffR<V> lane_bcast(AffR<V> v, int l) { return {lane_bcast(v.a, l), lane_bcast(v.b, l)}; }
+template <class V> __device__ __forceinline__ AffC<V> lane_bcast(AffC<V> v, int l)
+{
+    return {lane_bcast(v.ar, l), lane_bcast(v.ai, l), lane_bcast(v.br, l), lane_bcast(v.bi, l)};
+}
+
+// Exclusive block scan of affine maps (composition, earlier threads first).
+// lds must hold NW+1 entries.  Returns the prefix; *total = all threads composed.
+// Wave totals go through LDS once; every wave then scans the NW totals in its
+// first lanes (DPP, NW <= 16) and picks its own prefix with a lane broadcast.
 template <int NT, class A>
 __device__ A block_scan_excl(A v, A* lds, A* total)
 {
     constexpr int NW = NT / 64;
+    static_assert(NW <= 16, "wave totals are scanned within one DPP row");
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const A id = aff_id((A*)nullptr);
     const A inc = wave_scan_incl(v);
     if (lane == 63) lds[wid] = inc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        A run = id;
-        for (int w = 0; w < NW; ++w) {
-            A t = lds[w];
-            lds[w] = run;
-            run = compose(run, t);
-        }
-        lds[NW] = run;
-    }
-    __syncthreads();
-    A exc = shfl_up(inc, 1);
-    if (lane == 0) exc = id;
-    A res = compose(lds[wid], exc);
-    *total = lds[NW];
-    __syncthreads();
+    A t = lane < NW ? lds[lane] : id;
+    if constexpr (NW > 1) t = compose(dpp_aff<0x111, 0xf>(t), t);
+    if constexpr (NW > 2) t = compose(dpp_aff<0x112, 0xf>(t), t);
+    if constexpr (NW > 4) t = compose(dpp_aff<0x114, 0xf>(t), t);
+    if constexpr (NW > 8) t = compose(dpp_aff<0x118, 0xf>(t), t);
+    const A wpre = wid > 0 ? lane_bcast(t, wid - 1) : id;
+    *total = lane_bcast(t, NW - 1);
+    A exc = dpp_aff<0x138, 0xf>(inc);      // wave_shr:1, lane 0 gets the identity
+    const A res = compose(wpre, exc);
+    __syncthreads();                       // lds may be reused by the next scan
     return res;
 }
 
@@ -225,22 +242,38 @@ __device__ double block_max(double v, double* lds)
     return r;
 }
 
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_max(double v)
+{
+    return fmax(v, dpp_f<CTRL, ROWMASK>(v, v));   // masked-off lanes see their own value
+}
+// maximum over the wave, valid in lane 63 (DPP max-scan)
+__device__ __forceinline__ double wave_max_last(double v)
+{
+    v = dpp_max<0x111, 0xf>(v);
+    v = dpp_max<0x112, 0xf>(v);
+    v = dpp_max<0x114, 0xf>(v);
+    v = dpp_max<0x118, 0xf>(v);
+    v = dpp_max<0x142, 0xa>(v);
+    v = dpp_max<0x143, 0xc>(v);
+    return v;
+}
+
 // block-wide max of two values at once (one barrier pair); lds: 2*NW doubles
 template <int NT>
 __device__ void block_max2(double& a, double& b, double* lds)
 {
     constexpr int NW = NT / 64;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        a = fmax(a, __shfl_xor(a, d));
-        b = fmax(b, __shfl_xor(b, d));
-    }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) { lds[wid] = a; lds[NW + wid] = b; }
+    a = wave_max_last(a);
+    b = wave_max_last(b);
+    if (lane == 63) { lds[wid] = a; lds[NW + wid] = b; }
     __syncthreads();
-    a = lds[0];
-    b = lds[NW];
-    for (int w = 1; w < NW; ++w) { a = fmax(a, lds[w]); b = fmax(b, lds[NW + w]); }
+    double x = lane < NW ? lds[lane] : lds[0], y = lane < NW ? lds[NW + lane] : lds[NW];
+    x = wave_max_last(x);
+    y = wave_max_last(y);
+    a = lane_bcast(x, 63);
+    b = lane_bcast(y, 63);
     __syncthreads();
 }
 
