@@ -85,8 +85,9 @@ int launch_fft_lines(Prec p, int nt, const void* in, void* out, int nlines, int 
 int launch_line_diag(Prec p, int nt, const void* field, int nlines, const double* wfilter, double tstep,
                      const NLArgs& a, double* flux, double* maxI, double* rhomax, void* et, double* rho, hipStream_t st);
 // QDHT GEMM: out = A * in, A = diag(m1) C diag(1/m1) padded to rp x rp.
-// amat: FP64/FP32 transposed matrix, or the hi/lo bf16 pair for MIXED.
-int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* amat_lo, const void* in, void* out,
+// amat: FP64/FP32 transposed matrix, or for MIXED the bf16 hi/lo split in
+// MFMA fragment order (see the plan set-up).
+int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* in, void* out,
                 const LstepArgs* lstep, hipStream_t st);
 // element-wise linear step from a k-space increment (calculate_Lstep)
 int launch_lstep(Prec p, int rp, int nt, const void* b1k, const LstepArgs& la, hipStream_t st);

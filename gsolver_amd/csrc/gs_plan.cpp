@@ -96,7 +96,7 @@ struct gs_plan {
     hipEvent_t ev[5] = {};
 
     DevBuf field, b1, b2, fk, tmp;       // [rp][T] in field precision
-    DevBuf amat, amat_lo, at64;          // QDHT operators
+    DevBuf amat, at64;                   // QDHT operators
     DevBuf tw, tw64, zpow, pf, kerr, ionrate, k2, lrate, lgfac, avgw;
     DevBuf linemax, status, diag, et, rhot, scratch64a, scratch64b;
     std::vector<DevBuf> wfilt;
@@ -202,18 +202,22 @@ void build_plan(gs_plan* p)
         std::vector<float> f(At.begin(), At.end());
         upload(p->amat, f.data(), f.size() * 4);
     } else {
-        // split-BF16: hi = bf16(a), lo = bf16(a - hi), tiled [m/128][k/32][128][32]
-        // (one 8 KiB block per row tile and K step of the GEMM)
-        std::vector<uint16_t> hi(A.size()), lo(A.size());
+        // split-BF16: hi = bf16(a), lo = bf16(a - hi), in the MFMA fragment order of
+        // the GEMM: [m/128][k/32][k slice (k%32)/16][32-row block][hi, lo][lane][8],
+        // lane = m%32 + 32 ((k%16)/8), element k%8 (v_mfma_f32_32x32x16_bf16 A operand)
+        std::vector<uint16_t> fr((size_t)2 * rp * rp);
+        const int nk = rp / 32;
         for (int m = 0; m < rp; ++m)
             for (int k = 0; k < rp; ++k) {
                 const float a = (float)A[(size_t)m * rp + k];
-                const size_t t = ((size_t)(m / 128) * (rp / 32) + k / 32) * 4096 + (m % 128) * 32 + k % 32;
-                hi[t] = bf16_rn_host(a);
-                lo[t] = bf16_rn_host(a - bf16_to_float(hi[t]));
+                const uint16_t h = bf16_rn_host(a), l = bf16_rn_host(a - bf16_to_float(h));
+                const int mt = m / 128, rbb = (m % 128) / 32, step = k / 32, kk = (k % 32) / 16;
+                const int lane = m % 32 + 32 * ((k % 16) / 8), e = k % 8;
+                const size_t base = (((((size_t)mt * nk + step) * 2 + kk) * 4 + rbb) * 2) * 64 * 8;
+                fr[base + lane * 8 + e] = h;
+                fr[base + 64 * 8 + lane * 8 + e] = l;
             }
-        upload(p->amat, hi.data(), hi.size() * 2);
-        upload(p->amat_lo, lo.data(), lo.size() * 2);
+        upload(p->amat, fr.data(), fr.size() * 2);
     }
 
     // ---- tables
@@ -370,7 +374,7 @@ void build_plan(gs_plan* p)
         if (p->scratch64b.n < bytes) p->scratch64b.alloc(bytes);
         HCK(hipMemsetAsync(p->scratch64a.p, 0, bytes, p->st));
         HCK(hipMemcpyAsync(p->scratch64a.p, sp.data(), n * 16, hipMemcpyHostToDevice, p->st));
-        LCK(gs::launch_qdht(gs::kFP64, rp, nt, p->at64.p, nullptr, p->scratch64a.p, p->scratch64b.p, nullptr, p->st));
+        LCK(gs::launch_qdht(gs::kFP64, rp, nt, p->at64.p, p->scratch64a.p, p->scratch64b.p, nullptr, p->st));
         LCK(gs::launch_convert(p->prec, p->scratch64b.as<double>(), p->b2.p, (size_t)rp * nt, 1, p->st));
         sync(p);
     }
@@ -396,10 +400,10 @@ void linear_half(gs_plan* p, double dz)
 {
     p->ls.dz = dz;
     HCK(hipEventRecord(p->ev[2], p->st));
-    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, p->prec == gs::kFP64 ? p->at64.p : p->amat.p, p->amat_lo.p, p->b1.p,
+    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, p->prec == gs::kFP64 ? p->at64.p : p->amat.p, p->b1.p,
                         nullptr, &p->ls, p->st));
     HCK(hipEventRecord(p->ev[3], p->st));
-    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, p->prec == gs::kFP64 ? p->at64.p : p->amat.p, p->amat_lo.p, p->fk.p,
+    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, p->prec == gs::kFP64 ? p->at64.p : p->amat.p, p->fk.p,
                         p->field.p, nullptr, p->st));
 }
 
@@ -766,7 +770,7 @@ int gs_qdht(gs_plan* p, int buffer)
     return guarded([&]() {
         if (!p) return fail(GS_E_ARG, "null plan");
         void* b = p->buf(buffer);
-        LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, p->prec == gs::kFP64 ? p->at64.p : p->amat.p, p->amat_lo.p, b,
+        LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, p->prec == gs::kFP64 ? p->at64.p : p->amat.p, b,
                             p->tmp.p, nullptr, p->st));
         HCK(hipMemcpyAsync(b, p->tmp.p, p->field_bytes(), hipMemcpyDeviceToDevice, p->st));
         sync(p);
@@ -896,9 +900,9 @@ int gs_fixed_steps(gs_plan* p, double dz, int n, double* ms4)
             HCK(hipEventRecord(e[1], p->st));
             LCK(gs::launch_reduce_max(p->s.NR, p->linemax.as<double>(), p->status.as<gs::DevStatus>(), p->st));
             HCK(hipEventRecord(e[2], p->st));
-            LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->amat_lo.p, p->b1.p, nullptr, &p->ls, p->st));
+            LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->b1.p, nullptr, &p->ls, p->st));
             HCK(hipEventRecord(e[3], p->st));
-            LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->amat_lo.p, p->fk.p, p->field.p, nullptr, p->st));
+            LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->fk.p, p->field.p, nullptr, p->st));
             HCK(hipEventRecord(e[4], p->st));
         }
         check_status(p, nullptr);     // synchronises; NaN / blast flags of any of the steps

@@ -129,30 +129,30 @@ __device__ __forceinline__ void split2(float v0, float v1, uint32_t* hi, uint32_
 
 // BF16 3-pass, large tile: 128 rows x 128 complex columns per workgroup,
 // 4 waves of 64 x 64 complex, K step 32 (two MFMA k-slices of 16).
-// Software pipeline over double-buffered LDS, one barrier per K step:
-// while the MFMAs of step k read buffer k&1, the bf16 split of step k+1
-// (loaded from global during step k-1) is written to the other buffer and
-// the global loads of step k+2 are in flight.  B is loaded one column per
-// lane (16 k rows, coalesced across lanes) and stored k-contiguous.
+// The kernel is bound by LDS bandwidth, so only B goes through LDS: the
+// QDHT operator A is stored in MFMA fragment order (one coalesced 1 KiB
+// load per 32 x 16 fragment) and read from L2 straight into registers one K
+// step ahead.  B (the field, FP32) is loaded one column per lane (16 k rows,
+// coalesced across lanes) two K steps ahead, split into bf16 hi/lo and stored
+// k-contiguous in a double-buffered LDS tile; one barrier per K step.
 #ifndef GS_GEMM_SCHED
 #define GS_GEMM_SCHED 1
 #endif
 namespace t128 {
 constexpr int BM = 128, BNC = 128, BK = 32, LD = BK + 8;
-constexpr int PLANE = BM * LD;          // elements of one [128][LD] plane
-constexpr int BUF = 6 * PLANE;          // A hi, A lo, B hi re, B hi im, B lo re, B lo im
+constexpr int PLANE = BNC * LD;         // elements of one [128 columns][LD] plane
+constexpr int BUF = 4 * PLANE;          // B hi re, B hi im, B lo re, B lo im
 constexpr size_t SMEM = (size_t)2 * BUF * sizeof(uint16_t);
-// global loads of one K step: A hi/lo (4 x 16 B), 16 complex B values of one column
-__device__ __forceinline__ void gload(const uint16_t* gAh, const uint16_t* gAl, const C2<float>* gX, int NC, int step,
-                                      int nk, u32x4 (&ga)[4], float (&xr)[16], float (&xi)[16])
+// A fragment index: [m tile][k step][k slice (2)][32-row block (4)][hi/lo][lane][8]
+__host__ __device__ constexpr size_t afrag(int mt, int nk, int step, int kk, int rbb, int hl)
+{
+    return ((((((size_t)mt * nk + step) * 2 + kk) * 4 + rbb) * 2 + hl) * 64) * 8;
+}
+// global loads of one K step: 16 complex B values of one column
+__device__ __forceinline__ void gload_b(const C2<float>* gX, int NC, int step, int nk, float (&xr)[16], float (&xi)[16])
 {
     step = step < nk ? step : nk - 1;      // past the end: reload the last step (never used)
-    const size_t ka = (size_t)step * BK, kt = (size_t)step * BM * BK;
-    ga[0] = *(const u32x4*)(gAh + kt);
-    ga[1] = *(const u32x4*)(gAh + kt + 8);
-    ga[2] = *(const u32x4*)(gAl + kt);
-    ga[3] = *(const u32x4*)(gAl + kt + 8);
-    const C2<float>* xp = gX + ka * NC;
+    const C2<float>* xp = gX + (size_t)step * BK * NC;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const C2<float> v = xp[(size_t)j * NC];
@@ -160,14 +160,21 @@ __device__ __forceinline__ void gload(const uint16_t* gAh, const uint16_t* gAl, 
         xi[j] = v.y;
     }
 }
-// split of one K step into the LDS buffer b
-__device__ __forceinline__ void sstore(uint16_t* b, int oA, int oB, const u32x4 (&ga)[4], const float (&xr)[16],
-                                       const float (&xi)[16])
+// the wave's A fragments of one K step: [kk][rb][hi/lo]
+__device__ __forceinline__ void gload_a(const uint16_t* A, int mi, int nk, int step, int rbb0, int lane, v8bf (&a)[2][2][2])
 {
-    *(u32x4*)(b + oA) = ga[0];
-    *(u32x4*)(b + oA + 8) = ga[1];
-    *(u32x4*)(b + PLANE + oA) = ga[2];
-    *(u32x4*)(b + PLANE + oA + 8) = ga[3];
+    step = step < nk ? step : nk - 1;
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int hl = 0; hl < 2; ++hl)
+                a[kk][rb][hl] = *(const v8bf*)(A + afrag(mi, nk, step, kk, rbb0 + rb, hl) + lane * 8);
+}
+// split of one K step of B into the LDS buffer b
+__device__ __forceinline__ void sstore_b(uint16_t* b, int oB, const float (&xr)[16], const float (&xi)[16])
+{
     uint32_t hr[8], lr[8], hi[8], li[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -186,7 +193,7 @@ __device__ __forceinline__ void sstore(uint16_t* b, int oA, int oB, const u32x4 
 }  // namespace t128
 
 template <bool LSTEP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_qdht_bf16x3_t128(const uint16_t* __restrict__ Ah, const uint16_t* __restrict__ Al,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_qdht_bf16x3_t128(const uint16_t* __restrict__ Afr,
                                                           const C2<float>* __restrict__ X, C2<float>* __restrict__ Y,
                                                           int K, int NC, LstepArgs la)
 {
@@ -200,20 +207,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     const int m0 = mi * BM, n0 = ni * BNC;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
-    const int arow = tid >> 1, ak = (tid & 1) * 16;      // A loader: row, k half
     const int bcol = tid & 127, bk = (tid >> 7) * 16;     // B loader: column, k half
     const int cl = lane & 31, kq = 8 * (lane >> 5);
-
-    // per-thread LDS offsets within a buffer (elements)
-    const int oA = arow * LD + ak;                        // + PLANE for A lo
-    const int oB = 2 * PLANE + bcol * LD + bk;            // + PLANE per B plane
-    const int rA = (wm + cl) * LD + kq;                   // + rb*32*LD, + PLANE for lo, + kk
-    const int rB = 2 * PLANE + (wn + cl) * LD + kq;       // + plane*PLANE, + cb*32*LD, + kk
-
-    // A is stored tiled, [m tile][k step][BM][BK], so a K step of a row tile is 8 KiB contiguous
-    const size_t atile = (size_t)mi * (K / BK) * (BM * BK) + arow * BK + ak;
-    const uint16_t* gAh = Ah + atile;
-    const uint16_t* gAl = Al + atile;
+    const int oB = bcol * LD + bk;                        // + PLANE per B plane
+    const int rB = (wn + cl) * LD + kq;                   // + plane*PLANE, + cb*32*LD, + kk
+    const int rbb0 = wm / 32;
     const C2<float>* gX = X + (size_t)bk * NC + n0 + bcol;
     const int nk = K / BK;
 
@@ -227,53 +225,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
                 for (int r = 0; r < 16; ++r) acc[a][b][c][r] = 0.f;
 
-    // all MFMA operands of one K step (both k-slices) come out of LDS before
-    // the first MFMA, so the MFMA stream never waits on an LDS read
-    struct Ops {
-        v8bf ah[2][2], al[2][2], bh[2][2][2], bl[2][2][2];   // [kk][rb] / [kk][pt][cb]
+    // the B operands of one K step come out of LDS before the first MFMA
+    struct OpsB {
+        v8bf h[2][2][2], l[2][2][2];   // [kk][pt][cb]
     };
-    auto read_ops = [&](const uint16_t* b, Ops& o) {
+    auto read_b = [&](const uint16_t* b, OpsB& o) {
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int kk = 16 * q;
-#pragma unroll
-            for (int rb = 0; rb < 2; ++rb) {
-                o.ah[q][rb] = *(const v8bf*)(b + rA + rb * 32 * LD + kk);
-                o.al[q][rb] = *(const v8bf*)(b + PLANE + rA + rb * 32 * LD + kk);
-            }
+        for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
             for (int pt = 0; pt < 2; ++pt)
 #pragma unroll
                 for (int cb = 0; cb < 2; ++cb) {
-                    o.bh[q][pt][cb] = *(const v8bf*)(b + rB + pt * PLANE + cb * 32 * LD + kk);
-                    o.bl[q][pt][cb] = *(const v8bf*)(b + rB + (2 + pt) * PLANE + cb * 32 * LD + kk);
+                    o.h[kk][pt][cb] = *(const v8bf*)(b + rB + pt * PLANE + cb * 32 * LD + 16 * kk);
+                    o.l[kk][pt][cb] = *(const v8bf*)(b + rB + (2 + pt) * PLANE + cb * 32 * LD + 16 * kk);
                 }
-        }
     };
-    auto mfma_ops = [&](const Ops& o) {
+    auto mfma_ops = [&](const v8bf (&a)[2][2][2], const OpsB& o) {
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-#pragma unroll
-            for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-                for (int pt = 0; pt < 2; ++pt)
-#pragma unroll
-                    for (int cb = 0; cb < 2; ++cb)
-                        acc[rb][pt][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(o.al[q][rb], o.bh[q][pt][cb], acc[rb][pt][cb], 0, 0, 0);
+        for (int kk = 0; kk < 2; ++kk) {
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
                 for (int pt = 0; pt < 2; ++pt)
 #pragma unroll
                     for (int cb = 0; cb < 2; ++cb)
-                        acc[rb][pt][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(o.ah[q][rb], o.bl[q][pt][cb], acc[rb][pt][cb], 0, 0, 0);
+                        acc[rb][pt][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[kk][rb][1], o.h[kk][pt][cb], acc[rb][pt][cb], 0, 0, 0);
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
                 for (int pt = 0; pt < 2; ++pt)
 #pragma unroll
                     for (int cb = 0; cb < 2; ++cb)
-                        acc[rb][pt][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(o.ah[q][rb], o.bh[q][pt][cb], acc[rb][pt][cb], 0, 0, 0);
+                        acc[rb][pt][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[kk][rb][0], o.l[kk][pt][cb], acc[rb][pt][cb], 0, 0, 0);
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+                    for (int cb = 0; cb < 2; ++cb)
+                        acc[rb][pt][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[kk][rb][0], o.h[kk][pt][cb], acc[rb][pt][cb], 0, 0, 0);
         }
     };
     // interleave the 48 MFMAs of a step with the split VALU, the LDS writes
@@ -282,55 +272,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         if constexpr (GS_GEMM_SCHED == 0) return;
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            if constexpr (GS_GEMM_SCHED == 2) {
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
-                __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);   // VALU
-                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);   // VMEM read
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
-                __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);   // DS write
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
-            } else if constexpr (GS_GEMM_SCHED == 3) {
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                __builtin_amdgcn_sched_group_barrier(0x020, 2, 0);
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
-            } else {
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
-                __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);   // VALU
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);   // DS write
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                __builtin_amdgcn_sched_group_barrier(0x020, 2, 0);   // VMEM read
-            }
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);   // VALU
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);   // DS write
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x020, 2, 0);   // VMEM read
         }
     };
 
     uint16_t* const buf0 = sm;
     uint16_t* const buf1 = sm + BUF;
-    u32x4 ga[4], gna[4];
     float xr[16], xi[16], nxr[16], nxi[16];
-    gload(gAh, gAl, gX, NC, 0, nk, ga, xr, xi);
-    sstore(buf0, oA, oB, ga, xr, xi);
-    gload(gAh, gAl, gX, NC, 1, nk, ga, xr, xi);
+    v8bf a0[2][2][2], a1[2][2][2];
+    gload_b(gX, NC, 0, nk, xr, xi);
+    sstore_b(buf0, oB, xr, xi);
+    gload_b(gX, NC, 1, nk, xr, xi);
+    gload_a(Afr, mi, nk, 0, rbb0, lane, a0);
     __syncthreads();
-    Ops ops;
+    OpsB ops;
     for (int k = 0; k < nk; k += 2) {
-        read_ops(buf0, ops);
+        read_b(buf0, ops);
         __builtin_amdgcn_sched_barrier(0);
-        gload(gAh, gAl, gX, NC, k + 2, nk, gna, nxr, nxi);
-        mfma_ops(ops);
-        sstore(buf1, oA, oB, ga, xr, xi);
+        gload_b(gX, NC, k + 2, nk, nxr, nxi);
+        gload_a(Afr, mi, nk, k + 1, rbb0, lane, a1);
+        mfma_ops(a0, ops);
+        sstore_b(buf1, oB, xr, xi);
         schedule();
         __syncthreads();
-        read_ops(buf1, ops);
+        read_b(buf1, ops);
         __builtin_amdgcn_sched_barrier(0);
-        gload(gAh, gAl, gX, NC, k + 3, nk, ga, xr, xi);
-        mfma_ops(ops);
-        sstore(buf0, oA, oB, gna, nxr, nxi);
+        gload_b(gX, NC, k + 3, nk, xr, xi);
+        gload_a(Afr, mi, nk, k + 2, rbb0, lane, a0);
+        mfma_ops(a1, ops);
+        sstore_b(buf0, oB, nxr, nxi);
         schedule();
         __syncthreads();
     }
@@ -483,7 +458,7 @@ __global__ void k_avg_spectrum(int nr, int nt, const C2<R>* __restrict__ f, cons
     out[w] = acc;
 }
 
-int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* amat_lo, const void* in, void* out,
+int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* in, void* out,
                 const LstepArgs* ls, hipStream_t st)
 {
     LstepArgs la = ls ? *ls : LstepArgs{};
@@ -504,8 +479,8 @@ int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* amat_lo, c
         auto kf = k_qdht_bf16x3_t128<false>;
         HIPCHK(hipFuncSetAttribute((const void*)kt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBf16T128Smem));
         HIPCHK(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBf16T128Smem));
-        if (ls) hipLaunchKernelGGL(kt, grid, dim3(256), kBf16T128Smem, st, (const uint16_t*)amat, (const uint16_t*)amat_lo, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
-        else hipLaunchKernelGGL(kf, grid, dim3(256), kBf16T128Smem, st, (const uint16_t*)amat, (const uint16_t*)amat_lo, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
+        if (ls) hipLaunchKernelGGL(kt, grid, dim3(256), kBf16T128Smem, st, (const uint16_t*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
+        else hipLaunchKernelGGL(kf, grid, dim3(256), kBf16T128Smem, st, (const uint16_t*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
     }
     HIPCHK(hipGetLastError());
     return 0;
