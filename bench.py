@@ -111,17 +111,32 @@ def accuracy_check(gsa, workloads, work, z_max, planes):
     return res
 
 
-def roofline(kind, ms, R, T, prec):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_1024x4096.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (scaled by the copy-kernel calibration), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)["kernels"][kernel]["hbm_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def roofline(kind, ms, R, T, prec, kernel=None):
     if kind == "qdht":
         flops = 4.0 * R * R * T                 # one transform: real R x R times 2T real columns
         ach = flops / (ms * 1e-3) / 1e12
         peak = PEAK_TFLOPS[prec]
         return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
-                "traffic": None, "kernel": "k_qdht (%s), %.3f GFLOP algorithmic per launch" % (prec, flops / 1e9)}
+                "traffic": measured_traffic(kernel) if kernel else None,
+                "kernel": "%s, %.3f GFLOP algorithmic per launch" % (kernel or "k_qdht (%s)" % prec, flops / 1e9)}
     nbytes = 16.0 * R * T                       # read FIELD + write BIGBUFFER1 (FP32 complex)
     ach = nbytes / (ms * 1e-3) / 1e9
+    k = kernel or "k_nl_rk4<PolMix>"
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": None, "kernel": "k_nl_rk4, %.1f MiB algorithmic per launch" % (nbytes / 2**20)}
+            "traffic": measured_traffic(k), "kernel": "%s, %.1f MiB algorithmic per launch" % (k, nbytes / 2**20)}
 
 
 def main():
@@ -181,7 +196,8 @@ def main():
     R, T = m["R"], m["T"]
     qdht_ms = 0.5 * (km["qdht_lstep"] + km["qdht_back"])
     dominant = "nl" if km["nl_rk4"] >= km["qdht_lstep"] + km["qdht_back"] else "qdht"
-    roof = roofline("qdht" if dominant == "qdht" else "nl", qdht_ms if dominant == "qdht" else km["nl_rk4"], R, T, "mixed")
+    roof = (roofline("qdht", qdht_ms, R, T, "mixed", "k_qdht_bf16x3_t128<false>") if dominant == "qdht"
+            else roofline("nl", km["nl_rk4"], R, T, "mixed", "k_nl_rk4<PolMix>"))
 
     acc = None
     if not args.no_accuracy and not args.no_fp64:
@@ -213,10 +229,11 @@ def main():
         "accuracy_march_vs_fp64": acc,
         "kernel_ms_per_step": {p: res[p]["kernel_ms"] for p in res},
         "roofline": roof,
-        "roofline_qdht": roofline("qdht", qdht_ms, R, T, "mixed"),
-        "roofline_qdht_fp32": roofline("qdht", 0.5 * (res["fp32"]["kernel_ms"]["qdht_lstep"] +
-                                                      res["fp32"]["kernel_ms"]["qdht_back"]), R, T, "fp32"),
-        "roofline_nl": roofline("nl", km["nl_rk4"], R, T, "mixed"),
+        "roofline_qdht": roofline("qdht", km["qdht_back"], R, T, "mixed", "k_qdht_bf16x3_t128<false>"),
+        "roofline_qdht_lstep": roofline("qdht", km["qdht_lstep"], R, T, "mixed", "k_qdht_bf16x3_t128<true>"),
+        "roofline_qdht_fp32": roofline("qdht", res["fp32"]["kernel_ms"]["qdht_back"], R, T, "fp32",
+                                       "k_qdht_f32<false>"),
+        "roofline_nl": roofline("nl", km["nl_rk4"], R, T, "mixed", "k_nl_rk4<PolMix>"),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
