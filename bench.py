@@ -98,7 +98,8 @@ def accuracy_check(gsa, workloads, work, z_max, planes):
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b)) if np.linalg.norm(b) > 0 else float(np.abs(a).max())
     ref = out["fp64"]
     res = {"z_planes_m": ref["z"][1:], "steps": {k: v["steps"] for k, v in out.items()},
-           "rejected": {k: v["rejected"] for k, v in out.items()}}
+           "rejected": {k: v["rejected"] for k, v in out.items()},
+           "adaptive_steps_per_s": {k: v["steps"] / v["wall_s"] for k, v in out.items()}}
     for prec in ("mixed", "fp32"):
         o = out[prec]
         res[prec] = {

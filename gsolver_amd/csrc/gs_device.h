@@ -25,6 +25,27 @@ struct DevStatus {
     int fallbacks;
 };
 
+// Step-controller state of the device-driven march (propagator.cu:125-266
+// and the main loop of solver_main.cpp:202-225 without host round trips).
+// One z-step attempt ("slot") is: NL RK4 at dz -> controller -> linear half at
+// dz_lin if accepted -> finish.  Kernels of a slot exit at once when state != 0
+// (and the linear half when the attempt was rejected).
+struct DevCtl {
+    double dz;          // ZSTEP: dz of the next NL attempt
+    double z;           // CURRENT_Z
+    double z_plane;     // ZNET[n_Z], the plane the march is heading for
+    double dz_floor;    // 1e-10 (ZNET[n_Z] - ZNET[n_Z - 1]): below it the run has collapsed
+    double tol;         // MAX_TOLERANCE
+    double dz_lin;      // dz of this slot's linear half (the new ZSTEP)
+    long long steps;    // STEP_N (accepted and rejected attempts)
+    long long rejects;  // n_reject
+    long long accepted; // accepted steps since the counter was last cleared by the host
+    int accept;         // this slot: 1 accepted, 0 rejected
+    int state;          // 0 running, 1 reached z_plane, 2 error
+    int err;            // 1 step collapse, 2 NaN intensity, 3 blast
+    int pad;
+};
+
 // Scalars and table pointers the NL kernels read.  Table pointers refer to
 // device arrays in the plan's field precision unless noted.
 struct NLArgs {
@@ -44,6 +65,7 @@ struct NLArgs {
     const void* tw;             // e^{-2 pi i k/T}, k < T (C2<R>)
     DevStatus* status;
     int nl_e;                   // samples per thread of the RK4 kernel (0: default)
+    const DevCtl* ctl;          // device-driven march: dz from ctl->dz, skip when ctl->state != 0
 };
 
 // Linear step (propagator.cu:570-625) from per-point set-up tables:
@@ -57,6 +79,7 @@ struct LstepArgs {
     const void* gfac;           // [rp][T], field precision
     void* b2;                   // BIGBUFFER2 (C2<R>), updated in place
     void* fk;                   // FIELD in k-space (C2<R>), written
+    const DevCtl* ctl;          // device-driven march: dz = ctl->dz_lin; skip unless accepted
 };
 
 // sqrt(1+x) - 1 as the 20-term Taylor sum of extmath.h:245-252, Horner form
@@ -88,11 +111,15 @@ int launch_line_diag(Prec p, int nt, const void* field, int nlines, const double
 // amat: FP64/FP32 transposed matrix, or for MIXED the bf16 hi/lo split in
 // MFMA fragment order (see the plan set-up).
 int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* in, void* out,
-                const LstepArgs* lstep, hipStream_t st);
+                const LstepArgs* lstep, hipStream_t st, const DevCtl* gate = nullptr);
 // element-wise linear step from a k-space increment (calculate_Lstep)
 int launch_lstep(Prec p, int rp, int nt, const void* b1k, const LstepArgs& la, hipStream_t st);
 // max of the per-line maxima -> status
 int launch_reduce_max(int nr, const double* linemax, DevStatus* st_out, hipStream_t st);
+// device-driven march: maxima -> accept / reject, next dz, clamp to the plane (controller),
+// and the bookkeeping after the linear half (finish)
+int launch_controller(int nr, const double* linemax, DevStatus* status, DevCtl* ctl, hipStream_t st);
+int launch_finish(DevCtl* ctl, hipStream_t st);
 // conversions between the host FP64 layout and the device field precision
 int launch_convert(Prec p, const double* src64, void* dst, size_t n_complex, int to_device, hipStream_t st);
 // average spectrum (dump_types.cpp:429-457): out[w] = XMAX^2/2 sum_r |F_rw|^2 x_r (x_r - x_{r-1})

@@ -391,6 +391,10 @@ __global__ __launch_bounds__((Shape<Pol, T, E>::NT), GS_NL_WAVES) void k_nl_rk4(
     using R = typename Pol::R;
     constexpr int NT = Shape<Pol, T, E>::NT;
     constexpr bool ROWS_LDS = RK4Rows<Pol, T>::in_lds;
+    if (a.ctl) {
+        if (a.ctl->state != 0) return;
+        dz = a.ctl->dz;
+    }
     extern __shared__ __align__(16) unsigned char smem_raw[];
     LineSmem<Pol, T, E>& sm = *reinterpret_cast<LineSmem<Pol, T, E>*>(smem_raw);
     const int line = blockIdx.x, tid = threadIdx.x;

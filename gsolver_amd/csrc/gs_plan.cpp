@@ -108,6 +108,12 @@ struct gs_plan {
     int n_Z = 0;
     double timings[4] = {0, 0, 0, 0};
 
+    // device-driven march: controller state, its pinned host mirror, and the
+    // captured graph of one step attempt
+    DevBuf ctl;
+    gs::DevCtl* ctl_host = nullptr;
+    hipGraphExec_t slot_exec = nullptr;
+
     void* buf(int id)
     {
         switch (id) {
@@ -448,6 +454,88 @@ void propagator(gs_plan* p)
     }
 }
 
+// ---- device-driven march
+// one step attempt: NL RK4 at ctl.dz, controller, linear half at ctl.dz_lin
+// when accepted, bookkeeping; every kernel reads its dz and gate from ctl
+void enqueue_slot(gs_plan* p)
+{
+    gs::DevCtl* c = p->ctl.as<gs::DevCtl>();
+    gs::NLArgs a = p->nl;
+    a.ctl = c;
+    LCK(gs::launch_nl_rk4(p->prec, p->s.NT, p->field.p, p->b1.p, 0.0, a, p->linemax.as<double>(), p->st));
+    LCK(gs::launch_controller(p->s.NR, p->linemax.as<double>(), p->status.as<gs::DevStatus>(), c, p->st));
+    gs::LstepArgs l = p->ls;
+    l.ctl = c;
+    const void* A = p->prec == gs::kFP64 ? p->at64.p : p->amat.p;
+    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->b1.p, nullptr, &l, p->st));
+    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->fk.p, p->field.p, nullptr, p->st, c));
+    LCK(gs::launch_finish(c, p->st));
+}
+
+// the slot captured once into a graph, so a step costs one graph launch
+void launch_slot(gs_plan* p)
+{
+    if (!p->slot_exec && !std::getenv("GS_NO_GRAPH")) {
+        hipGraph_t g = nullptr;
+        HCK(hipStreamBeginCapture(p->st, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue_slot(p);
+        } catch (...) {
+            (void)hipStreamEndCapture(p->st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HCK(hipStreamEndCapture(p->st, &g));
+        const hipError_t e = hipGraphInstantiate(&p->slot_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HCK(e);
+    }
+    if (p->slot_exec) HCK(hipGraphLaunch(p->slot_exec, p->st));
+    else enqueue_slot(p);
+}
+
+// host mirror <- device controller state
+void pull_ctl(gs_plan* p)
+{
+    HCK(hipMemcpyAsync(p->ctl_host, p->ctl.p, sizeof(gs::DevCtl), hipMemcpyDeviceToHost, p->st));
+    sync(p);
+    const gs::DevCtl& h = *p->ctl_host;
+    p->ZSTEP = h.dz;
+    p->CURRENT_Z = h.z;
+    p->STEP_N = (long)h.steps;
+    p->n_reject = (long)h.rejects;
+}
+
+// arm the controller for the plane ZNET[n_Z] from the host state
+void arm_ctl(gs_plan* p)
+{
+    if (!p->ctl.p) {
+        p->ctl.alloc(sizeof(gs::DevCtl));
+        HCK(hipHostMalloc((void**)&p->ctl_host, sizeof(gs::DevCtl), 0));
+    }
+    const auto& Z = p->s.ZNET;
+    const int nz = std::max(p->n_Z, 1);
+    gs::DevCtl& h = *p->ctl_host;
+    h = gs::DevCtl{};
+    h.dz = p->ZSTEP;
+    h.z = p->CURRENT_Z;
+    h.z_plane = Z[p->n_Z];
+    h.dz_floor = 1e-10 * (Z[nz] - Z[nz - 1]);
+    h.tol = p->s.MAX_TOLERANCE;
+    h.steps = p->STEP_N;
+    h.rejects = p->n_reject;
+    HCK(hipMemcpyAsync(p->ctl.p, p->ctl_host, sizeof(gs::DevCtl), hipMemcpyHostToDevice, p->st));
+}
+
+[[noreturn]] void ctl_error(gs_plan* p, int err)
+{
+    HCK(hipMemsetAsync(p->status.p, 0, sizeof(gs::DevStatus), p->st));
+    sync(p);
+    if (err == 2) throw gs::Error(GS_E_PHYSICS, "float_type photoionization_function(...): Intensity is nan!");
+    if (err == 3) throw gs::Error(GS_E_PHYSICS, "Attention!!!! Intensity is too big! Keep away, blast is possible.");
+    throw gs::Error(GS_E_PHYSICS, "Minimum Z step size reached, collapse is possible!");
+}
+
 // ---- .udf output (dump_types.cpp:25-72)
 struct Udf {
     std::FILE* f = nullptr;
@@ -605,6 +693,8 @@ void gs_plan_destroy(gs_plan* p)
     if (p->st) (void)hipStreamSynchronize(p->st);
     for (auto& e : p->ev)
         if (e) (void)hipEventDestroy(e);
+    if (p->slot_exec) (void)hipGraphExecDestroy(p->slot_exec);
+    if (p->ctl_host) (void)hipHostFree(p->ctl_host);
     hipStream_t st = p->st;
     delete p;
     if (st) (void)hipStreamDestroy(st);
@@ -830,6 +920,26 @@ int gs_march(gs_plan* p, const char* outdir, long max_steps)
         open_dumps(p, outdir ? outdir : ".", &ds);
         const auto& Z = p->s.ZNET;
         long steps = 0;
+        if (std::getenv("GS_HOST_MARCH")) {
+            // host-driven controller: one synchronisation per step attempt
+            for (;;) {
+                if (p->CURRENT_Z >= Z[p->n_Z]) {
+                    do_dumps(p, &ds);
+                    if (p->n_Z == p->s.NZ - 1) return 0;
+                    p->n_Z++;
+                }
+                if (max_steps > 0 && steps >= max_steps) return 1;
+                if (p->n_Z == 0) throw gs::Error(GS_E_PHYSICS, "CURRENT_Z is below the first dump plane");
+                propagator(p);
+                p->STEP_N++;
+                p->CURRENT_Z += p->ZSTEP;
+                ++steps;
+            }
+        }
+        // device-driven controller: step attempts are enqueued in chunks and the
+        // host reads the controller state once per chunk; attempts after the
+        // plane (or after an error) are no-ops
+        constexpr long kChunk = 32;
         for (;;) {
             if (p->CURRENT_Z >= Z[p->n_Z]) {
                 do_dumps(p, &ds);
@@ -838,10 +948,20 @@ int gs_march(gs_plan* p, const char* outdir, long max_steps)
             }
             if (max_steps > 0 && steps >= max_steps) return 1;
             if (p->n_Z == 0) throw gs::Error(GS_E_PHYSICS, "CURRENT_Z is below the first dump plane");
-            propagator(p);
-            p->STEP_N++;
-            p->CURRENT_Z += p->ZSTEP;
-            ++steps;
+            arm_ctl(p);
+            for (;;) {
+                const long want = max_steps > 0 ? std::min(kChunk, max_steps - steps) : kChunk;
+                for (long k = 0; k < want; ++k) launch_slot(p);
+                pull_ctl(p);
+                const gs::DevCtl& h = *p->ctl_host;
+                steps += (long)h.accepted;
+                if (h.state == 2) ctl_error(p, h.err);
+                if (h.accepted > 0) {
+                    // re-arm keeps the plane; clears the accepted counter
+                    if (h.state == 1 || (max_steps > 0 && steps >= max_steps)) break;
+                    arm_ctl(p);
+                }
+            }
         }
     });
 }

@@ -42,6 +42,10 @@ __global__ __launch_bounds__(256) void k_qdht_f32(const float* __restrict__ At, 
                                                   C2<float>* __restrict__ Y, int K, int NC, LstepArgs la)
 {
     constexpr int BM = 128, BNC = 64, BK = 16, PA = 32, PB = 32;
+    if (la.ctl) {
+        if (la.ctl->state != 0 || !la.ctl->accept) return;
+        la.dz = la.ctl->dz_lin;
+    }
     __shared__ float As[BK][BM + PA];
     __shared__ float Br[BK][BNC + PB];
     __shared__ float Bi[BK][BNC + PB];
@@ -200,6 +204,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     using namespace t128;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     uint16_t* const sm = reinterpret_cast<uint16_t*>(smem_raw);
+    if (la.ctl) {
+        if (la.ctl->state != 0 || !la.ctl->accept) return;
+        la.dz = la.ctl->dz_lin;
+    }
 
     const int M = K;
     int mi, ni;
@@ -344,6 +352,10 @@ __global__ __launch_bounds__(256) void k_qdht_f64(const double* __restrict__ At,
                                                   C2<double>* __restrict__ Y, int K, int NC, LstepArgs la)
 {
     constexpr int BM = 64, BNC = 32, BK = 16, PA = 8, PB = 8;
+    if (la.ctl) {
+        if (la.ctl->state != 0 || !la.ctl->accept) return;
+        la.dz = la.ctl->dz_lin;
+    }
     __shared__ double As[BK][BM + PA];
     __shared__ double Br[BK][BNC + PB];
     __shared__ double Bi[BK][BNC + PB];
@@ -429,6 +441,56 @@ __global__ void k_reduce_max(int nr, const double* __restrict__ lm, DevStatus* s
     }
 }
 
+// propagator.cu:139-160 on the device: reject if dz^2 maxNL2 > tol^2 maxA2 (the
+// attempt counts in STEP_N and n_reject), dz_new = 0.5 tol sqrt(maxA2 / maxNL2),
+// collapse below dz_floor, clamp to the plane; the NaN / blast status of the
+// attempt ends the run as the reference's throws do
+__global__ void k_controller(int nr, const double* __restrict__ lm, DevStatus* status, DevCtl* ctl)
+{
+    __shared__ double red[64];
+    if (ctl->state != 0) return;
+    double a = 0, b = 0;
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) {
+        a = fmax(a, lm[2 * i]);
+        b = fmax(b, lm[2 * i + 1]);
+    }
+    const double A = block_max<1024>(a, red);
+    const double B = block_max<1024>(b, red + 32);
+    if (threadIdx.x != 0) return;
+    status->maxA2 = A;
+    status->maxNL2 = B;
+    if (status->flags & kFlagNaN) { ctl->state = 2; ctl->err = 2; return; }
+    if (status->flags & kFlagBlast) { ctl->state = 2; ctl->err = 3; return; }
+    const double dz = ctl->dz, tol = ctl->tol;
+    const bool repeat = dz * dz * B > tol * tol * A;
+    if (repeat) {
+        ctl->steps += 1;
+        ctl->rejects += 1;
+    }
+    double dzn = 0.5 * tol * sqrt(A / B);
+    if (dzn < ctl->dz_floor || !(dzn > 0)) {
+        ctl->dz = dzn;
+        ctl->state = 2;
+        ctl->err = 1;
+        return;
+    }
+    dzn = fmin(dzn, ctl->z_plane - ctl->z);
+    ctl->dz = dzn;
+    ctl->dz_lin = dzn;
+    ctl->accept = repeat ? 0 : 1;
+}
+
+// after the linear half: CURRENT_Z += ZSTEP, STEP_N++ (solver_main.cpp:217-220);
+// stop at the plane
+__global__ void k_finish(DevCtl* ctl)
+{
+    if (ctl->state != 0 || !ctl->accept) return;
+    ctl->z += ctl->dz_lin;
+    ctl->steps += 1;
+    ctl->accepted += 1;
+    if (ctl->z >= ctl->z_plane) ctl->state = 1;
+}
+
 template <class R>
 __global__ void k_convert(const double* __restrict__ src, C2<R>* __restrict__ dst, size_t n)
 {
@@ -459,9 +521,10 @@ __global__ void k_avg_spectrum(int nr, int nt, const C2<R>* __restrict__ f, cons
 }
 
 int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* in, void* out,
-                const LstepArgs* ls, hipStream_t st)
+                const LstepArgs* ls, hipStream_t st, const DevCtl* gate)
 {
     LstepArgs la = ls ? *ls : LstepArgs{};
+    if (!ls) la.ctl = gate;
     if (p == kFP64) {
         if (rp % 64 || nt % 32) return -1001;
         const dim3 grid((rp / 64) * (nt / 32));
@@ -493,6 +556,20 @@ int launch_lstep(Prec p, int rp, int nt, const void* b1k, const LstepArgs& la, h
     if (p == kFP64) hipLaunchKernelGGL(k_lstep<PolF64>, grid, dim3(256), 0, st, (const C2<double>*)b1k, nt, n, la);
     else if (p == kFP32) hipLaunchKernelGGL(k_lstep<PolF32>, grid, dim3(256), 0, st, (const C2<float>*)b1k, nt, n, la);
     else hipLaunchKernelGGL(k_lstep<PolMix>, grid, dim3(256), 0, st, (const C2<float>*)b1k, nt, n, la);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int launch_controller(int nr, const double* linemax, DevStatus* status, DevCtl* ctl, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_controller, dim3(1), dim3(1024), 0, st, nr, linemax, status, ctl);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int launch_finish(DevCtl* ctl, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1), 0, st, ctl);
     HIPCHK(hipGetLastError());
     return 0;
 }
