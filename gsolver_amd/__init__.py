@@ -59,6 +59,7 @@ def lib():
         L.gs_fft_lines.argtypes = [vp, D, D, ip, ip]
         L.gs_last_timings.argtypes = [vp, D]
         L.gs_fixed_steps.argtypes = [vp, dp, ip, D]
+        L.gs_march_many.argtypes = [C.POINTER(vp), ip, C.POINTER(cp), C.POINTER(ip)]
         _lib = L
     return _lib
 
@@ -87,6 +88,21 @@ def setup_table(cfg, name, n, cplx=False, overrides=()):
     finally:
         os.chdir(cwd)
     return out
+
+
+def march_many(plans, outdirs):
+    """gs_march of several plans at once (their steps overlap on the device)."""
+    n = len(plans)
+    hs = (C.c_void_p * n)(*[p.h for p in plans])
+    ds = (C.c_char_p * n)(*[os.path.abspath(d).encode() for d in outdirs])
+    rcs = (C.c_int * n)()
+    cwd = os.getcwd()
+    os.chdir(plans[0].cfg_dir)
+    try:
+        _check(lib().gs_march_many(hs, n, ds, rcs))
+    finally:
+        os.chdir(cwd)
+    return list(rcs)
 
 
 class Plan:

@@ -13,7 +13,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <thread>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -388,6 +392,58 @@ void build_plan(gs_plan* p)
     p->s.field_t.shrink_to_fit();
 }
 
+// ---- autotuner of the line kernel's samples per thread
+// For FP32 / mixed lines of N_T >= 1024 the RK4 kernel has two shapes (8 or 16
+// samples per thread: 512 or 256 threads per line, different register and LDS
+// footprints).  The first plan of a (precision, N_T, N_R) in the process times
+// both on its initial field and keeps the faster; GS_NL_E overrides.
+void autotune_nl(gs_plan* p)
+{
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int>, int> cache;
+    gs::NLArgs& a = p->nl;
+    if (a.nl_e != 0 || p->prec == gs::kFP64 || p->s.NT < 1024) return;
+    const auto key = std::make_tuple((int)p->prec, p->s.NT, p->s.NR);
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            a.nl_e = it->second;
+            return;
+        }
+    }
+    hipEvent_t e0, e1;
+    HCK(hipEventCreate(&e0));
+    HCK(hipEventCreate(&e1));
+    int best = 8;
+    float best_ms = 1e30f;
+    for (int e : {8, 16}) {
+        a.nl_e = e;
+        float ms = 1e30f;
+        for (int rep = 0; rep < 3; ++rep) {
+            HCK(hipEventRecord(e0, p->st));
+            LCK(gs::launch_nl_rk4(p->prec, p->s.NT, p->field.p, p->b1.p, p->ZSTEP, a, p->linemax.as<double>(), p->st));
+            HCK(hipEventRecord(e1, p->st));
+            sync(p);
+            float t = 0;
+            HCK(hipEventElapsedTime(&t, e0, e1));
+            if (rep > 0) ms = std::min(ms, t);
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = e;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HCK(hipMemsetAsync(p->status.p, 0, sizeof(gs::DevStatus), p->st));
+    HCK(hipMemsetAsync(p->b1.p, 0, p->field_bytes(), p->st));
+    sync(p);
+    a.nl_e = best;
+    std::lock_guard<std::mutex> g(mu);
+    cache[key] = best;
+}
+
 // ---- one NL attempt + controller pieces
 void nl_step(gs_plan* p, double dz, double* maxA2, double* maxNL2)
 {
@@ -682,6 +738,7 @@ int gs_plan_create(const char* cfg_path, int nover, const char* const* over, int
         gs::Config cfg(cfg_path, ov);
         p->s = gs::build_setup(cfg, workdir ? workdir : "");
         build_plan(p.get());
+        autotune_nl(p.get());
         *out = p.release();
         return 0;
     });
@@ -963,6 +1020,77 @@ int gs_march(gs_plan* p, const char* outdir, long max_steps)
                 }
             }
         }
+    });
+}
+
+int gs_march_many(gs_plan* const* plans, int n, const char* const* outdirs, int* rcs)
+{
+    return guarded([&]() {
+        if (!plans || n <= 0) return fail(GS_E_ARG, "gs_march_many: no plans");
+        for (int i = 0; i < n; ++i)
+            if (!plans[i]) return fail(GS_E_ARG, "gs_march_many: null plan");
+        // every run keeps its own stream, controller and dumps; the host
+        // round-robins over the runs, enqueues a chunk of step attempts for each
+        // run that is not busy and polls the chunks' completion events, so the
+        // runs' kernels overlap on the device
+        constexpr long kChunk = 16;
+        struct Run {
+            DumpSet ds;
+            hipEvent_t done = nullptr;
+            bool busy = false, active = true;
+        };
+        std::vector<Run> runs(n);
+        struct Guard {
+            std::vector<Run>& r;
+            ~Guard() { for (auto& x : r) if (x.done) (void)hipEventDestroy(x.done); }
+        } guard{runs};
+        for (int i = 0; i < n; ++i) {
+            open_dumps(plans[i], outdirs && outdirs[i] ? outdirs[i] : ".", &runs[i].ds);
+            HCK(hipEventCreateWithFlags(&runs[i].done, hipEventDisableTiming));
+            if (rcs) rcs[i] = 1;
+        }
+        int left = n;
+        while (left > 0) {
+            bool progressed = false;
+            for (int i = 0; i < n; ++i) {
+                Run& r = runs[i];
+                gs_plan* p = plans[i];
+                if (!r.active) continue;
+                if (r.busy) {
+                    const hipError_t q = hipEventQuery(r.done);
+                    if (q == hipErrorNotReady) continue;
+                    HCK(q);
+                    r.busy = false;
+                    progressed = true;
+                    const gs::DevCtl& h = *p->ctl_host;
+                    p->ZSTEP = h.dz;
+                    p->CURRENT_Z = h.z;
+                    p->STEP_N = (long)h.steps;
+                    p->n_reject = (long)h.rejects;
+                    if (h.state == 2) ctl_error(p, h.err);
+                }
+                const auto& Z = p->s.ZNET;
+                if (p->CURRENT_Z >= Z[p->n_Z]) {
+                    do_dumps(p, &r.ds);
+                    if (p->n_Z == p->s.NZ - 1) {
+                        r.active = false;
+                        --left;
+                        if (rcs) rcs[i] = 0;
+                        continue;
+                    }
+                    p->n_Z++;
+                }
+                if (p->n_Z == 0) throw gs::Error(GS_E_PHYSICS, "CURRENT_Z is below the first dump plane");
+                arm_ctl(p);
+                for (long k = 0; k < kChunk; ++k) launch_slot(p);
+                HCK(hipMemcpyAsync(p->ctl_host, p->ctl.p, sizeof(gs::DevCtl), hipMemcpyDeviceToHost, p->st));
+                HCK(hipEventRecord(r.done, p->st));
+                r.busy = true;
+                progressed = true;
+            }
+            if (!progressed) std::this_thread::yield();
+        }
+        return 0;
     });
 }
 

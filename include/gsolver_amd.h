@@ -120,6 +120,13 @@ int gs_fixed_step(gs_plan* p, double dz, double* maxA2, double* maxNL2);
  * steps when max_steps > 0 (returns 1 then; 0 at the last plane). */
 int gs_march(gs_plan* p, const char* outdir, long max_steps);
 
+/* gs_march for n independent runs at once (energy / width scans, uneven
+ * grids): each plan marches to its last plane with its own controller and
+ * dumps (outdirs[i], NULL = "."), the runs' step attempts overlapping on the
+ * device.  rcs (may be NULL) receives 0 per finished run.  No counterpart in
+ * the reference, which runs one configuration per process. */
+int gs_march_many(gs_plan* const* plans, int n, const char* const* outdirs, int* rcs);
+
 /* Dump-side reductions of the current FIELD, per radial line (N_R each):
  * flux = TSTEP * sum_t |E|^2, maxI = max_t |E|^2, rhomax = max_t rho(t),
  * with E = IFFT(FIELD)/N_T (dump_types.cpp:123-165, propagator.cu:935-946).

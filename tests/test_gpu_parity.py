@@ -237,3 +237,40 @@ def test_device_driven_march_matches_host_controller(tmp_path, prec, monkeypatch
     assert (h["steps"], h["rej"], h["z"]) == (g["steps"], g["rej"], g["z"])
     for k in ("field", "b2", "f", "pm"):
         np.testing.assert_array_equal(h[k], g[k])
+
+
+def test_march_many_matches_single_runs(tmp_path):
+    """gs_march_many over runs with uneven grids (N_R, N_T) and pulse energies,
+    mixed precision: every run's dumps and final field equal those of the same
+    run marched alone."""
+    specs = [dict(N_R="64", N_T="512", ENERGY="8e-6"), dict(N_R="128", N_T="1024", ENERGY="4e-6"),
+             dict(N_R="64", N_T="256", ENERGY="12e-6")]
+    cfgs = []
+    for i, sp in enumerate(specs):
+        kw = dict(HOT, N_Z="3", Z_MAX="6e-5", MAX_TOLERANCE="3e-3", DUMPS_N="2", DUMPTYPE0="flux",
+                  DUMP_FILENAME0="m_%s_f.udf", DUMPTYPE1="plasma_max", DUMP_FILENAME1="m_%s_pm.udf")
+        kw.update(sp)
+        cfgs.append(write_cfg(tmp_path / ("c%d.txt" % i), **kw))
+    single = []
+    for i, c in enumerate(cfgs):
+        d = tmp_path / ("s%d" % i)
+        d.mkdir()
+        p = gsa.Plan(c, precision="mixed")
+        assert p.march(str(d)) == 0
+        single.append((p.get(), p.scalar("STEP_N"), oracle.load_udf(str(d / "m__f.udf"))[1],
+                       oracle.load_udf(str(d / "m__pm.udf"))[1]))
+        p.close()
+    plans = [gsa.Plan(c, precision="mixed") for c in cfgs]
+    dirs = []
+    for i in range(len(cfgs)):
+        d = tmp_path / ("b%d" % i)
+        d.mkdir()
+        dirs.append(str(d))
+    assert gsa.march_many(plans, dirs) == [0] * len(cfgs)
+    for i, p in enumerate(plans):
+        f, steps, fl, pm = single[i]
+        print("many", i, steps, p.scalar("STEP_N"))
+        assert p.scalar("STEP_N") == steps
+        np.testing.assert_array_equal(p.get(), f)
+        np.testing.assert_array_equal(oracle.load_udf(dirs[i] + "/m__f.udf")[1], fl)
+        np.testing.assert_array_equal(oracle.load_udf(dirs[i] + "/m__pm.udf")[1], pm)
