@@ -125,6 +125,51 @@ def measured_traffic(kernel):
         return None
 
 
+def batch_check(gsa, workloads, work, nruns, z_max, seed=20261202):
+    """Energy / width scan over uneven grids: nruns runs with pulse energy
+    log-uniform in 0.5-10 uJ, w0 uniform in 40-150 um, N_R in {128, 256, 512}
+    and N_T in {512, 1024, 2048} (seeded), the 0.5 mm / 400 fs fused-silica
+    set-up, each marched to z_max in mixed precision with the reference
+    controller.  Timed once run by run and once all together
+    (gs_march_many); reports aggregate accepted z-steps/s of both and the
+    largest difference between a run's batched and single field."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    d = os.path.join(work, "batch")
+    os.makedirs(d, exist_ok=True)
+    cfgs = []
+    for i in range(nruns):
+        e = float(np.exp(rng.uniform(np.log(0.5e-6), np.log(10e-6))))
+        w0 = float(rng.uniform(40e-6, 150e-6))
+        nr = int(rng.choice([128, 256, 512]))
+        nt = int(rng.choice([512, 1024, 2048]))
+        cfgs.append(workloads.write_config(d, "c0", "run%02d" % i, N_R=nr, N_T=nt, ENERGY="%.6e" % e,
+                                           DIAMETER_FWHM="%.6e" % (w0 * np.sqrt(2 * np.log(2))),
+                                           N_Z=2, Z_MAX=z_max))
+    single, t_single, steps = [], 0.0, 0
+    for c in cfgs:
+        p = gsa.Plan(c, precision="mixed")
+        t0 = time.perf_counter()
+        p.march(d)
+        t_single += time.perf_counter() - t0
+        single.append(p.get())
+        steps += int(p.scalar("STEP_N") - p.scalar("n_reject"))
+        p.close()
+    plans = [gsa.Plan(c, precision="mixed") for c in cfgs]
+    t0 = time.perf_counter()
+    gsa.march_many(plans, [d] * nruns)
+    t_batch = time.perf_counter() - t0
+    diff = max(float(np.abs(p.get() - f).max()) for p, f in zip(plans, single))
+    for p in plans:
+        p.close()
+    out = {"runs": nruns, "z_max_m": z_max, "accepted_steps": steps,
+           "batched_steps_per_s": steps / t_batch, "single_steps_per_s": steps / t_single,
+           "speedup": t_single / t_batch, "max_abs_field_diff_batched_vs_single": diff}
+    print("[bench] batch: %d runs, %d steps, batched %.0f steps/s, one by one %.0f steps/s, max diff %g"
+          % (nruns, steps, out["batched_steps_per_s"], out["single_steps_per_s"], diff), file=sys.stderr, flush=True)
+    return out
+
+
 def roofline(kind, ms, R, T, prec, kernel=None):
     if kind == "qdht":
         flops = 4.0 * R * R * T                 # one transform: real R x R times 2T real columns
@@ -149,6 +194,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-accuracy", action="store_true", help="skip the adaptive-march accuracy check")
+    ap.add_argument("--batch-runs", type=int, default=16, help="runs of the energy/width scan (0: skip)")
+    ap.add_argument("--batch-zmax", type=float, default=5e-6, help="distance of each scan run in m")
     ap.add_argument("--accuracy-zmax", type=float, default=5e-5,
                     help="distance of the accuracy-check march in m (two dump planes)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -204,6 +251,8 @@ def main():
     if not args.no_accuracy and not args.no_fp64:
         acc = accuracy_check(gsa, workloads, work, args.accuracy_zmax, 2)
 
+    batch = batch_check(gsa, workloads, work, args.batch_runs, args.batch_zmax) if args.batch_runs > 0 else None
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -228,6 +277,7 @@ def main():
         "mixed_over_fp32": m["steps_per_s"] / res["fp32"]["steps_per_s"],
         "field_rel_l2_vs_fp64": err or None,
         "accuracy_march_vs_fp64": acc,
+        "energy_width_scan": batch,
         "kernel_ms_per_step": {p: res[p]["kernel_ms"] for p in res},
         "roofline": roof,
         "roofline_qdht": roofline("qdht", km["qdht_back"], R, T, "mixed", "k_qdht_bf16x3_t128<false>"),

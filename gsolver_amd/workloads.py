@@ -60,15 +60,16 @@ def workload_keys(name):
     raise KeyError(name)
 
 
-def write_config(directory, name, **overrides):
-    """Write <directory>/<name>.txt and the fused-silica .rawdisp beside it."""
+def write_config(directory, name, out_name=None, **overrides):
+    """Write <directory>/<out_name or name>.txt for workload `name` (keys
+    overridden by the keyword arguments) and the fused-silica .rawdisp beside it."""
     os.makedirs(directory, exist_ok=True)
     rd = os.path.join(directory, "fused_silica_sellmeier.rawdisp")
     if not os.path.exists(rd):
         write_sellmeier_rawdisp(rd)
     keys = workload_keys(name)
     keys.update({k: str(v) for k, v in overrides.items()})
-    path = os.path.join(directory, name + ".txt")
+    path = os.path.join(directory, (out_name or name) + ".txt")
     with open(path, "w") as f:
         for k, v in keys.items():
             f.write("%s %s\n" % (k, v))
