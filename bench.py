@@ -35,6 +35,8 @@ def run_precision(gsa, cfg, prec, steps, warmup, dz):
     back; the library synchronises the device before it returns, so the wall
     clock brackets complete device work."""
     p = gsa.Plan(cfg, precision=prec)
+    print("[bench] %s: line kernel %d samples per thread (autotuned)" % (prec, p.scalar("NL_SAMPLES_PER_THREAD")),
+          file=sys.stderr, flush=True)
     if warmup > 0:
         p.fixed_steps(dz, warmup)
     t0 = time.perf_counter()

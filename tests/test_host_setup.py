@@ -69,3 +69,13 @@ def test_noise_map_is_seeded_and_matches_oracle(tmp_path):
     f0 = oracle.Sim(cfg0).get("field")
     d = rel_l2(o1.get("field"), f0)
     assert 1e-3 < d < 2e-2
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with a message and status 2."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
