@@ -104,27 +104,16 @@ __device__ __forceinline__ void dft_small(C2<R>* v)
     }
 }
 
-// v[r] *= w^r, r = 1..RADIX-1, with w = tw[idx] (conjugated for SIGN > 0):
-// the powers w^(2^j) are loaded from the table (tw[2^j idx], all inside the
-// table since idx RADIX/2 < T) and the others are products of one loaded
-// power and one lower power (RADIX - 1 - log2 RADIX multiplications, chains of
-// at most log2 RADIX - 1 products).
-template <int RADIX, int SIGN, class R>
-__device__ __forceinline__ void twiddle_tab(C2<R>* v, const C2<R>* __restrict__ tw, int idx)
+// v[r] *= w^r for r = 1..RADIX-1.  The powers come from a product tree
+// (w^r = w^(r/2) w^(r - r/2), depth <= 4 for RADIX <= 16), so a pass needs
+// one twiddle load instead of RADIX-1 and the rounding error stays at a few ulp.
+template <int RADIX, class R>
+__device__ __forceinline__ void twiddle_pow(C2<R>* v, C2<R> w)
 {
     C2<R> p[RADIX];
+    p[1] = w;
 #pragma unroll
-    for (int b = 1; b < RADIX; b <<= 1) {
-        C2<R> w = tw[b * idx];
-        if (SIGN > 0) w.y = -w.y;
-        p[b] = w;
-    }
-#pragma unroll
-    for (int r = 3; r < RADIX; ++r) {
-        if ((r & (r - 1)) == 0) continue;
-        const int hb = 1 << (31 - __builtin_clz(r));
-        p[r] = cmul(p[hb], p[r - hb]);
-    }
+    for (int r = 2; r < RADIX; ++r) p[r] = cmul(p[r / 2], p[r - r / 2]);
 #pragma unroll
     for (int r = 1; r < RADIX; ++r) v[r] = cmul(v[r], p[r]);
 }

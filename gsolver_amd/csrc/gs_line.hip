@@ -70,7 +70,11 @@ __device__ __forceinline__ void spass(const C2<R>* __restrict__ src, C2<R>* __re
 #pragma unroll
             for (int r = 0; r < RADIX; ++r) v[r] = src[pidx(j + r * NB, E)];
             const int k = j & (Ns - 1);
-            if (Ns > 1) twiddle_tab<RADIX, SIGN>(v, tw, k * (T / (Ns * RADIX)));
+            if (Ns > 1) {
+                C2<R> w = tw[k * (T / (Ns * RADIX))];
+                if (SIGN > 0) w.y = -w.y;
+                twiddle_pow<RADIX>(v, w);
+            }
             dft_small<RADIX, SIGN>(v);
             const int d = (j - k) * RADIX + k;
 #pragma unroll
@@ -117,7 +121,7 @@ __device__ __forceinline__ void fft_fwd_acc(const C2<R> (&x)[E], const F (&fac)[
     C2<R> y[E];
 #pragma unroll
     for (int r = 0; r < E; ++r) y[r] = src[pidx(tid + r * NT, E)];
-    twiddle_tab<E, -1>(y, tw, tid);
+    twiddle_pow<E>(y, tw[tid]);
     dft_small<E, -1>(y);
 #pragma unroll
     for (int r = 0; r < E; ++r) {
