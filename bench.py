@@ -198,7 +198,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the adaptive-march accuracy check")
     ap.add_argument("--batch-runs", type=int, default=16, help="runs of the energy/width scan (0: skip)")
     ap.add_argument("--batch-zmax", type=float, default=5e-6, help="distance of each scan run in m")
-    ap.add_argument("--accuracy-zmax", type=float, default=5e-5,
+    ap.add_argument("--accuracy-zmax", type=float, default=1e-3,
                     help="distance of the accuracy-check march in m (two dump planes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy (reference kernel for the HBM byte counters)")

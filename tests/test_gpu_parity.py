@@ -274,3 +274,25 @@ def test_march_many_matches_single_runs(tmp_path):
         np.testing.assert_array_equal(p.get(), f)
         np.testing.assert_array_equal(oracle.load_udf(dirs[i] + "/m__f.udf")[1], fl)
         np.testing.assert_array_equal(oracle.load_udf(dirs[i] + "/m__pm.udf")[1], pm)
+
+
+def test_cli_drop_in_matches_library_march(tmp_path):
+    """The gsolver_amd executable (the reference's `<cfg> [NAME VALUE]...`
+    contract) writes the same .udf dumps as gs_march through the library."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(gsa.__file__), "gsolver_amd")
+    kw = dict(HOT, N_Z="3", Z_MAX="4e-5", MAX_TOLERANCE="3e-3", DUMPS_N="1", DUMPTYPE0="flux",
+              DUMP_FILENAME0="cli_%s_f.udf")
+    cfg = write_cfg(tmp_path / "c.txt", **kw)
+    r = subprocess.run([exe, cfg, "N_R", "64", "--precision", "mixed"], cwd=str(tmp_path), capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    cli = oracle.load_udf(str(tmp_path / "cli__f.udf"))[1]
+    d = tmp_path / "lib"
+    d.mkdir()
+    p = gsa.Plan(cfg, overrides=["N_R", "64"], precision="mixed")
+    assert p.march(str(d)) == 0
+    np.testing.assert_array_equal(oracle.load_udf(str(d / "cli__f.udf"))[1], cli)
+    bad = subprocess.run([exe, str(tmp_path / "missing.txt")], cwd=str(tmp_path), capture_output=True, text=True,
+                         timeout=120)
+    assert bad.returncode == 2 and "Error" in bad.stdout
