@@ -136,7 +136,9 @@ __device__ __forceinline__ void fft_fwd_acc(const C2<R> (&x)[E], const F (&fac)[
         }
         out[r] = cadd(out[r], g);
     }
-    __syncthreads();   // src may be overwritten by the next transform
+    // the next transform writes b0 first and passes a barrier before it writes
+    // b1, so only a result left in b0 needs a barrier here
+    if constexpr (Chain::out == 0) __syncthreads();
 }
 
 // inverse (SIGN +1) FFT, unnormalised: spectrum in strided layout -> time in chunk layout
@@ -157,7 +159,7 @@ __device__ __forceinline__ void fft_inv(const C2<R> (&y)[E], C2<R> (&x)[E], C2<R
     const C2<R>* src = Chain::out ? b1 : b0;
 #pragma unroll
     for (int e = 0; e < E; ++e) x[e] = src[pidx(tid * E + e, E)];
-    __syncthreads();
+    if constexpr (Chain::out == 0) __syncthreads();   // as in fft_fwd_acc
 }
 
 // ------------------------------------------------------------------ plasma
@@ -306,7 +308,7 @@ __device__ __forceinline__ void nl_rhs(const C2<typename Pol::R> (&x)[E], C2<typ
     fft_inv<R, T, NT, E>(x, Ev, sm.b0, sm.b1, tw);
     R lossf[E];
     R rho[E];
-    bool anyloss = false, has_rho = false;
+    bool anyloss = false, has_rho = false, ionizing;
     {
         W Ws[E];
         bool anyw = false;
@@ -318,7 +320,8 @@ __device__ __forceinline__ void nl_rhs(const C2<typename Pol::R> (&x)[E], C2<typ
             anyw |= Ws[e] > (W)0;
         }
         // avalanche and loss need W > 0 somewhere (or ambient carriers)
-        if (__syncthreads_or(anyw) || a.rho_init != 0.0) {
+        ionizing = __syncthreads_or(anyw) || a.rho_init != 0.0;
+        if (ionizing) {
             W cs[E];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
@@ -336,7 +339,7 @@ __device__ __forceinline__ void nl_rhs(const C2<typename Pol::R> (&x)[E], C2<typ
 
 #pragma unroll
     for (int r = 0; r < E; ++r) out[r] = C2<R>{0, 0};
-    if (__syncthreads_or(anyloss)) fft_fwd_acc<0, R, T, NT, E>(Ev, lossf, out, sm.b0, sm.b1, tw, nullptr);
+    if (ionizing && __syncthreads_or(anyloss)) fft_fwd_acc<0, R, T, NT, E>(Ev, lossf, out, sm.b0, sm.b1, tw, nullptr);
     if (has_rho) fft_fwd_acc<1, R, T, NT, E>(Ev, rho, out, sm.b0, sm.b1, tw, a.pf);
     if (!a.use_kerr) return;
 
