@@ -117,9 +117,11 @@ def test_nl_step_vs_oracle(tmp_path, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
-def test_fixed_steps_vs_oracle(tmp_path, prec):
-    """Three full z-steps (NL RK4, QDHT, fused linear step, QDHT back) at a fixed dz."""
-    p, o = pair(tmp_path, prec, **HOT)
+@pytest.mark.parametrize("nr", [64, 100])
+def test_fixed_steps_vs_oracle(tmp_path, prec, nr):
+    """Three full z-steps (NL RK4, QDHT, fused linear step, QDHT back) at a fixed dz;
+    N_R = 100 leaves 28 padding rows in the 128-row GEMM tile."""
+    p, o = pair(tmp_path, prec, **dict(HOT, N_R=str(nr)))
     for k in range(3):
         o.fixed_step(2e-6)
         p.fixed_step(2e-6)
