@@ -53,6 +53,10 @@ def lib(precision="fp64"):
         L.orc_lstep.argtypes = [vp]
         L.orc_propagator.argtypes = [vp]
         L.orc_fixed_step.argtypes = [vp, dp, D, D]
+        L.orc_fixed_step2.argtypes = [vp, dp, dp]
+        L.orc_nl_response_vjp.argtypes = [vp, D, D, D]
+        L.orc_nl_step_vjp.argtypes = [vp, D, D, dp, D]
+        L.orc_adjoint_march.argtypes = [vp, ip, D, D, D, D, D, D]
         L.orc_run.argtypes = [vp, cp, C.c_long]
         L.orc_fft.argtypes = [ip, D, D, ip]
         for f in ("orc_bessel_j0", "orc_bessel_j1", "orc_dawson", "orc_sqrtho"):
@@ -204,6 +208,46 @@ class Sim:
         if self.L.orc_fixed_step(self.h, dz, C.byref(a), C.byref(b)):
             self._err()
         return a.value, b.value
+
+    def fixed_step2(self, dz_rk, dz_lin):
+        if self.L.orc_fixed_step2(self.h, dz_rk, dz_lin):
+            self._err()
+
+    # ---- reverse mode (o_adjoint.c); cotangent of complex z: dL/dRe z + i dL/dIm z
+    def nl_response_vjp(self, a, lam):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        lam = np.ascontiguousarray(lam, dtype=np.complex128)
+        out = np.empty_like(a)
+        if self.L.orc_nl_response_vjp(self.h, _dptr(a.view(np.float64)), _dptr(lam.view(np.float64)),
+                                      _dptr(out.view(np.float64))):
+            self._err()
+        return out
+
+    def nl_step_vjp(self, A, g, dz):
+        A = np.ascontiguousarray(A, dtype=np.complex128)
+        g = np.ascontiguousarray(g, dtype=np.complex128)
+        out = np.empty_like(A)
+        if self.L.orc_nl_step_vjp(self.h, _dptr(A.view(np.float64)), _dptr(g.view(np.float64)), float(dz),
+                                  _dptr(out.view(np.float64))):
+            self._err()
+        return out
+
+    def adjoint_march(self, dz_rk, dz_lin, gfield, gb2=None):
+        """March len(dz_rk) fixed steps from the current state, then return the
+        cotangents (gfield0, gb2_0) of the initial FIELD and BIG2."""
+        dz_rk = np.ascontiguousarray(dz_rk, dtype=np.float64)
+        dz_lin = np.ascontiguousarray(dz_lin, dtype=np.float64)
+        gf = np.ascontiguousarray(gfield, dtype=np.complex128).reshape(self.R, self.T)
+        g0 = np.empty_like(gf)
+        b0 = np.empty_like(gf)
+        gbp = None
+        if gb2 is not None:
+            gb = np.ascontiguousarray(gb2, dtype=np.complex128).reshape(self.R, self.T)
+            gbp = _dptr(gb.view(np.float64))
+        if self.L.orc_adjoint_march(self.h, len(dz_rk), _dptr(dz_rk), _dptr(dz_lin), _dptr(gf.view(np.float64)),
+                                    gbp, _dptr(g0.view(np.float64)), _dptr(b0.view(np.float64))):
+            self._err()
+        return g0, b0
 
     def run(self, outdir, max_steps=0):
         cwd = os.getcwd()

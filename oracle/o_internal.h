@@ -126,6 +126,19 @@ struct orc_sim {
 /* error handling: the reference throws const char*; the oracle longjmps back
  * to the API entry that armed s->jb. */
 void o_throw(orc_sim* s, const char* msg);
+/* arms s->jb in an API entry: a throw returns failret from that entry */
+#define ARM(s, failret)            \
+    do {                           \
+        (s)->jb_armed = 1;         \
+        if (setjmp((s)->jb)) {     \
+            (s)->jb_armed = 0;     \
+            return failret;        \
+        }                          \
+    } while (0)
+#define DISARM(s) ((s)->jb_armed = 0)
+
+/* parallel for over [0, n) on s->nthreads threads; fn(s, tid, i0, i1, ctx) */
+void o_parallel(orc_sim* s, int n, void (*fn)(orc_sim*, int, int, int, void*), void* ctx);
 
 /* config access, process_input.cpp:297-482 semantics */
 int    o_cfg_find(orc_sim* s, const char* name, const char** val);
@@ -144,6 +157,7 @@ void   o_qdht_run_many(orc_sim* s, cplx_t* data);
 double o_keldysh_rate_ln(orc_sim* s, double lnI);
 real_t o_sqrtho(real_t x);
 
+void   o_fixed_step2(orc_sim* s, double dz_rk, double dz_lin);
 void   o_setup(orc_sim* s, const char* workdir);   /* o_setup.c */
 void   o_linebuf_alloc(o_linebuf* b, int nt);
 void   o_linebuf_free(o_linebuf* b);

@@ -59,7 +59,7 @@ static void* o_job_run(void* a)
     return NULL;
 }
 
-static void o_parallel(orc_sim* s, int n, void (*fn)(orc_sim*, int, int, int, void*), void* ctx)
+void o_parallel(orc_sim* s, int n, void (*fn)(orc_sim*, int, int, int, void*), void* ctx)
 {
     int nth = s->nthreads;
     if (nth <= 1 || n < 2) { fn(s, 0, 0, n, ctx); return; }
@@ -492,15 +492,6 @@ static int make_dumps(orc_sim* s, const char* outdir, o_dump** out)
 /* ========================================================================== */
 /* API                                                                         */
 
-#define ARM(s, failret)            \
-    do {                           \
-        (s)->jb_armed = 1;         \
-        if (setjmp((s)->jb)) {     \
-            (s)->jb_armed = 0;     \
-            return failret;        \
-        }                          \
-    } while (0)
-#define DISARM(s) ((s)->jb_armed = 0)
 
 static int read_tokens(const char* path, o_config* c)
 {
@@ -729,6 +720,30 @@ int orc_fixed_step(orc_sim* s, double dz, double* maxA2, double* maxNL2)
     o_qdht_run_many(s, s->FIELD);
     s->CURRENT_Z += dz;
     s->STEP_N++;
+    DISARM(s);
+    return 0;
+}
+
+/* one step with the NL RK4 at dz_rk and the linear step at dz_lin: the
+ * accepted attempt of propagator() after a step-size change (the NL step runs
+ * at the previous ZSTEP, the Lstep at the new one) */
+void o_fixed_step2(orc_sim* s, double dz_rk, double dz_lin)
+{
+    double a, b;
+    s->ZSTEP = dz_rk;
+    nl_step(s, &a, &b);
+    o_qdht_run_many(s, s->BIG1);
+    s->ZSTEP = dz_lin;
+    lstep(s);
+    o_qdht_run_many(s, s->FIELD);
+    s->CURRENT_Z += dz_lin;
+    s->STEP_N++;
+}
+
+int orc_fixed_step2(orc_sim* s, double dz_rk, double dz_lin)
+{
+    ARM(s, -1);
+    o_fixed_step2(s, dz_rk, dz_lin);
     DISARM(s);
     return 0;
 }

@@ -85,6 +85,20 @@ int    orc_nl_step(orc_sim* s, double* maxA2, double* maxNL2);  /* a2: FIELD -> 
 int    orc_lstep(orc_sim* s);                                   /* a11 (uses ZSTEP), b1 must be in k-space */
 int    orc_propagator(orc_sim* s);                              /* a1: one accepted step */
 int    orc_fixed_step(orc_sim* s, double dz, double* maxA2, double* maxNL2);  /* one step at a given dz, no controller */
+/* one step, NL RK4 at dz_rk and linear step at dz_lin (no controller) */
+int    orc_fixed_step2(orc_sim* s, double dz_rk, double dz_lin);
+
+/* ---- reverse mode (o_adjoint.c): exact transpose of the restated scheme ---- */
+/* one line: gin = J^H lam of calculate_NLresponse at `in` */
+int    orc_nl_response_vjp(orc_sim* s, const double* in, const double* lam, double* gin);
+/* [R][T]: gA = J^H g of the RK4 NL step at FIELD = A and step dz */
+int    orc_nl_step_vjp(orc_sim* s, const double* A, const double* g, double dz, double* gA);
+/* n fixed steps from the current FIELD / BIG2 (state left at the end), then the
+ * cotangents of the initial FIELD and BIG2 from those of the final ones
+ * (gb2 may be NULL: zero).  Cotangent of complex z: dL/dRe z + i dL/dIm z. */
+int    orc_adjoint_march(orc_sim* s, int n, const double* dz_rk, const double* dz_lin, const double* gfield,
+                         const double* gb2, double* gfield0, double* gb2_0);
+
 /* main loop (solver_main.cpp:202-225) with dumps written into outdir; stops
  * after max_steps cycles when max_steps > 0 (returns 1 then, 0 at Z_MAX). */
 int    orc_run(orc_sim* s, const char* outdir, long max_steps);
