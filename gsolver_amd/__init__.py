@@ -60,6 +60,8 @@ def lib():
         L.gs_last_timings.argtypes = [vp, D]
         L.gs_fixed_steps.argtypes = [vp, dp, ip, D]
         L.gs_march_many.argtypes = [C.POINTER(vp), ip, C.POINTER(cp), C.POINTER(ip)]
+        L.gs_adjoint_forward.argtypes = [vp, ip, D, D]
+        L.gs_adjoint_backward.argtypes = [vp, D, D, ip, D, D]
         _lib = L
     return _lib
 
@@ -219,6 +221,31 @@ class Plan:
         out = np.empty_like(x)
         _check(lib().gs_fft_lines(self.h, _d(x.view(np.float64)), _d(out.view(np.float64)), x.shape[0], sign))
         return out
+
+    # ---- reverse mode: cotangent of complex z is dL/dRe z + i dL/dIm z
+    def adjoint_forward(self, dz_rk, dz_lin):
+        """March len(dz_rk) fixed steps (NL RK4 at dz_rk[k], linear step at
+        dz_lin[k]) keeping FIELD before every step on the device."""
+        a = np.ascontiguousarray(dz_rk, dtype=np.float64)
+        b = np.ascontiguousarray(dz_lin, dtype=np.float64)
+        if a.shape != b.shape:
+            raise ValueError("dz_rk and dz_lin differ in length")
+        _check(lib().gs_adjoint_forward(self.h, len(a), _d(a), _d(b)))
+
+    def adjoint_backward(self, gfield, gb2=None, tie_b2=True):
+        """Cotangents (gfield0, gb2_0) of the initial FIELD and BIGBUFFER2 of the
+        recorded march from those of the final ones; with tie_b2 the BIGBUFFER2
+        part is also folded into gfield0 (initial BIGBUFFER2 = QDHT(FIELD))."""
+        g = np.ascontiguousarray(gfield, dtype=np.complex128).reshape(self.R, self.T)
+        gbp = None
+        if gb2 is not None:
+            gb = np.ascontiguousarray(gb2, dtype=np.complex128).reshape(self.R, self.T)
+            gbp = _d(gb.view(np.float64))
+        g0 = np.empty((self.R, self.T), dtype=np.complex128)
+        b0 = np.empty_like(g0)
+        _check(lib().gs_adjoint_backward(self.h, _d(g.view(np.float64)), gbp, 1 if tie_b2 else 0,
+                                         _d(g0.view(np.float64)), _d(b0.view(np.float64))))
+        return g0, b0
 
     def timings(self):
         t = np.zeros(4)

@@ -1,0 +1,61 @@
+"""Gradients through the z-march with torch.autograd.
+
+PhaseMarch maps the spectral phase phi(omega) applied to an input pulse to
+the field at the end of a march with a recorded step-size sequence:
+
+    A0(r, omega) = field0(r, omega) exp(i phi(omega)),  BIGBUFFER2 = QDHT(A0),
+    field = march(A0)  (gs_adjoint_forward),
+
+and its backward runs the transposed march on the GPU (gs_adjoint_backward):
+dL/dphi(omega) = sum_r Im(g0(r, omega) conj(A0(r, omega))), g0 the cotangent
+of A0.  The reference has no adjoint; step sizes are held fixed (the gradient
+of the march for the recorded dz sequence).  Torch's complex-gradient
+convention (dL/dRe z + i dL/dIm z for a real L) is the library's.
+"""
+import numpy as np
+import torch
+
+from . import B2, FIELD
+
+
+def record_steps(plan, n_steps):
+    """Advance `plan` by n_steps accepted steps of the reference controller
+    (gs_step) and return the (dz_rk, dz_lin) arrays that replay them: the NL
+    step of an accepted attempt runs at the previous ZSTEP, the linear step at
+    the new one."""
+    rk, lin = [], []
+    for _ in range(int(n_steps)):
+        plan.step()
+        rk.append(plan.scalar("LAST_DZ_NL"))
+        lin.append(plan.scalar("ZSTEP"))
+    return np.array(rk), np.array(lin)
+
+
+class PhaseMarch(torch.autograd.Function):
+    """field = PhaseMarch.apply(phi, plan, field0, dz_rk, dz_lin)
+
+    phi: real tensor [N_T]; field0: complex [N_R, N_T] (r, omega) numpy array;
+    returns the complex128 field [N_R, N_T] after the march (CPU tensor).  The
+    plan's state is overwritten.  One backward per forward (the plan keeps the
+    checkpoints of the last forward only)."""
+
+    @staticmethod
+    def forward(ctx, phi, plan, field0, dz_rk, dz_lin):
+        ph = phi.detach().to("cpu", torch.float64).numpy()
+        A0 = np.asarray(field0, dtype=np.complex128) * np.exp(1j * ph)[None, :]
+        plan.put(A0, FIELD)
+        plan.put(A0, B2)
+        plan.qdht(B2)
+        plan.adjoint_forward(dz_rk, dz_lin)
+        ctx.plan = plan
+        ctx.A0 = A0
+        ctx.phi_meta = (phi.dtype, phi.device)
+        return torch.from_numpy(plan.get(FIELD))
+
+    @staticmethod
+    def backward(ctx, grad_field):
+        g = grad_field.detach().to("cpu", torch.complex128).numpy()
+        g0, _ = ctx.plan.adjoint_backward(g, None, tie_b2=True)
+        gphi = np.imag(g0 * np.conj(ctx.A0)).sum(axis=0)
+        dtype, device = ctx.phi_meta
+        return torch.from_numpy(gphi).to(device=device, dtype=dtype), None, None, None, None

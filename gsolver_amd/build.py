@@ -12,7 +12,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
-SOURCES = ["gs_line.hip", "gs_qdht.hip", "gs_host.cpp", "gs_special.cpp", "gs_plan.cpp"]
+SOURCES = ["gs_line.hip", "gs_adjoint.hip", "gs_qdht.hip", "gs_host.cpp", "gs_special.cpp", "gs_plan.cpp"]
 LIB = os.path.join(HERE, "libgsolver_amd.so")
 EXE = os.path.join(HERE, "gsolver_amd")
 

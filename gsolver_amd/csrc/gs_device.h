@@ -122,6 +122,16 @@ int launch_controller(int nr, const double* linemax, DevStatus* status, DevCtl* 
 int launch_finish(DevCtl* ctl, hipStream_t st);
 // conversions between the host FP64 layout and the device field precision
 int launch_convert(Prec p, const double* src64, void* dst, size_t n_complex, int to_device, hipStream_t st);
+// reverse mode (gs_adjoint.hip).  Transposed RK4 NL step over a.nr lines: gfield =
+// J^H gb1 at FIELD = field and step dz; stage holds three planes of `plane`
+// complex elements (the recomputed stage inputs)
+int launch_nl_rk4_vjp(Prec p, int nt, const void* field, const void* gb1, void* gfield, void* stage, size_t plane,
+                      double dz, const NLArgs& a, hipStream_t st);
+// transposed linear step at la.dz: gb2 <- conj(P) (gb2 + gfk), gb1k <- dz g gb2
+// (masked points: gb2 passes through, gb1k = 0)
+int launch_lstep_adj(Prec p, int rp, int nt, const void* gfk, void* gb2, void* gb1k, const LstepArgs& la, hipStream_t st);
+// y += x over n complex values in the field precision
+int launch_axpy(Prec p, void* y, const void* x, size_t n_complex, hipStream_t st);
 // average spectrum (dump_types.cpp:429-457): out[w] = XMAX^2/2 sum_r |F_rw|^2 x_r (x_r - x_{r-1})
 int launch_avg_spectrum(Prec p, int nr, int nt, const void* field, const double* wts, double* out, hipStream_t st);
 

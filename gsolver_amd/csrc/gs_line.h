@@ -1,0 +1,392 @@
+// gs_line.h -- building blocks of the per-radial-line kernels (gs_line.hip,
+// gs_adjoint.hip): line shapes, LDS Stockham FFTs, the plasma scan and the
+// nonlinear RHS of one line (calculate_NLresponse, propagator.cu:629-783).
+//
+// One workgroup owns one radial line of N_T samples, E samples per thread.
+// Time-domain data live in "chunk" layout (thread j holds t = jE..jE+E-1,
+// which the plasma and Raman scans need), spectral data in the "strided"
+// layout (thread j holds omega = j + NT*r), which is exactly where a Stockham
+// FFT leaves its output after a final radix-E pass and where an inverse
+// Stockham FFT wants its input for a first radix-E pass.  So spectral data
+// never round-trip through LDS: the forward FFT ends in registers and the
+// inverse starts from them.  Passes ping-pong between two padded LDS buffers
+// (index n + n/E, which keeps the chunk accesses free of bank conflicts), one
+// barrier per pass.
+//
+// The plasma density (ionization.cpp:9-95) is a nonlinear recurrence in t;
+// it is solved by Newton iterations on the whole line, each an affine
+// block scan of the linearised Heun maps, until the update is at rounding
+// level, with a chunk-sequential exact fallback.  Lines without any
+// ionization skip it (rho stays at its initial value 0).  The Raman
+// convolution (propagator.cu:696-703, circular) is evaluated in closed form
+// from one complex affine scan: y = (1-fR) I + fR M Im(s (1 - z^T) + z^{n+1} s_{T-1}).
+#pragma once
+
+#include "gs_dev.h"
+
+#ifndef GS_NL_E32
+#define GS_NL_E32 8    // samples per thread of the FP32/mixed line kernels for N_T >= 1024
+#endif
+#ifndef GS_NL_WAVES
+#define GS_NL_WAVES 1  // minimum waves per SIMD the RK4 kernel is register-allocated for
+#endif
+
+// launcher dispatch over the supported line lengths (T_ the compile-time N_T)
+#define GS_T_SWITCH(nt, CALL)                                   \
+    switch (nt) {                                               \
+    case 256: { constexpr int T_ = 256; return CALL; }          \
+    case 512: { constexpr int T_ = 512; return CALL; }          \
+    case 1024: { constexpr int T_ = 1024; return CALL; }        \
+    case 2048: { constexpr int T_ = 2048; return CALL; }        \
+    case 4096: { constexpr int T_ = 4096; return CALL; }        \
+    default: return -1000;                                      \
+    }
+
+namespace gs {
+
+template <class Pol, int T>
+struct DefaultE {
+    using R = typename Pol::R;
+    static constexpr int v = T >= 1024 ? (sizeof(R) == 8 ? 8 : GS_NL_E32) : (T >= 512 ? 8 : 4);
+};
+
+template <class Pol, int T, int E_ = DefaultE<Pol, T>::v>
+struct Shape {
+    using R = typename Pol::R;
+    static constexpr int E = E_;
+    static constexpr int NT = T / E;
+    static constexpr int NW = NT / 64;
+    static constexpr int PADN = T + T / E;     // padded LDS buffer length
+    static_assert(NT >= 64 && NT <= 1024, "unsupported N_T");
+};
+
+__device__ __forceinline__ int pidx(int n, int E) { return n + n / E; }
+
+// largest radix <= E (so every thread has at least one butterfly) that divides M
+template <int M, int E> struct PickRadix {
+    static constexpr int v = (E >= 16 && M % 16 == 0) ? 16 : (E >= 8 && M % 8 == 0) ? 8 : (M % 4 == 0) ? 4 : 2;
+};
+
+// One Stockham pass src -> dst (both padded LDS buffers), no barrier.
+template <class R, int T, int NT, int E, int RADIX, int SIGN>
+__device__ __forceinline__ void spass(const C2<R>* __restrict__ src, C2<R>* __restrict__ dst,
+                                      const C2<R>* __restrict__ tw, int Ns)
+{
+    constexpr int NB = T / RADIX;
+    constexpr int PER = (NB + NT - 1) / NT;
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int j = threadIdx.x + b * NT;
+        if (NB % NT == 0 || j < NB) {
+            C2<R> v[RADIX];
+#pragma unroll
+            for (int r = 0; r < RADIX; ++r) v[r] = src[pidx(j + r * NB, E)];
+            const int k = j & (Ns - 1);
+            if (Ns > 1) {
+                C2<R> w = tw[k * (T / (Ns * RADIX))];
+                if (SIGN > 0) w.y = -w.y;
+                twiddle_pow<RADIX>(v, w);
+            }
+            dft_small<RADIX, SIGN>(v);
+            const int d = (j - k) * RADIX + k;
+#pragma unroll
+            for (int r = 0; r < RADIX; ++r) dst[pidx(d + r * Ns, E)] = v[r];
+        }
+    }
+}
+
+// The passes that cover M more points, ping-ponging b0 <-> b1 starting from
+// buffer P; PassChain<...>::out is the buffer holding the result.
+template <class R, int T, int NT, int E, int M, int SIGN, int P>
+struct PassChain {
+    static constexpr int RAD = PickRadix<M, E>::v;
+    using Next = PassChain<R, T, NT, E, M / RAD, SIGN, 1 - P>;
+    static constexpr int out = Next::out;
+    static __device__ __forceinline__ void run(C2<R>* b0, C2<R>* b1, const C2<R>* tw, int Ns)
+    {
+        spass<R, T, NT, E, RAD, SIGN>(P ? b1 : b0, P ? b0 : b1, tw, Ns);
+        __syncthreads();
+        Next::run(b0, b1, tw, Ns * RAD);
+    }
+};
+template <class R, int T, int NT, int E, int SIGN, int P>
+struct PassChain<R, T, NT, E, 1, SIGN, P> {
+    static constexpr int out = P;
+    static __device__ __forceinline__ void run(C2<R>*, C2<R>*, const C2<R>*, int) {}
+};
+
+// forward (SIGN -1) FFT of the chunk-layout signal fac[e]*x[e]; the spectrum
+// (strided layout) times a per-omega factor G is added to out, with G by MODE:
+//   0: G = 1     1: G = M[omega] (complex table)     2: G = -i K[omega] (real table)
+template <int MODE, class R, int T, int NT, int E, class F>
+__device__ __forceinline__ void fft_fwd_acc(const C2<R> (&x)[E], const F (&fac)[E], C2<R> (&out)[E], C2<R>* b0,
+                                            C2<R>* b1, const C2<R>* tw, const void* table)
+{
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < E; ++e) b0[pidx(tid * E + e, E)] = cscale(x[e], (R)fac[e]);
+    __syncthreads();
+    using Chain = PassChain<R, T, NT, E, T / E, -1, 0>;
+    Chain::run(b0, b1, tw, 1);
+    const C2<R>* src = Chain::out ? b1 : b0;
+    // last pass: radix E at Ns = T/E = NT, butterfly j = tid, output omega = tid + NT r
+    C2<R> y[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) y[r] = src[pidx(tid + r * NT, E)];
+    twiddle_pow<E>(y, tw[tid]);
+    dft_small<E, -1>(y);
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        C2<R> g;
+        if constexpr (MODE == 0) {
+            g = y[r];
+        } else if constexpr (MODE == 1) {
+            g = cmul(((const C2<R>*)table)[tid + r * NT], y[r]);
+        } else {
+            const R k = ((const R*)table)[tid + r * NT];
+            g = C2<R>{k * y[r].y, -k * y[r].x};
+        }
+        out[r] = cadd(out[r], g);
+    }
+    // the next transform writes b0 first and passes a barrier before it writes
+    // b1, so only a result left in b0 needs a barrier here
+    if constexpr (Chain::out == 0) __syncthreads();
+}
+
+// inverse (SIGN +1) FFT, unnormalised: spectrum in strided layout -> time in chunk layout
+template <class R, int T, int NT, int E>
+__device__ __forceinline__ void fft_inv(const C2<R> (&y)[E], C2<R> (&x)[E], C2<R>* b0, C2<R>* b1, const C2<R>* tw)
+{
+    const int tid = threadIdx.x;
+    // first pass: radix E at Ns = 1 over j = tid, inputs tid + NT r (registers), outputs tid*E + r
+    C2<R> v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) v[r] = y[r];
+    dft_small<E, +1>(v);
+#pragma unroll
+    for (int r = 0; r < E; ++r) b0[pidx(tid * E + r, E)] = v[r];
+    __syncthreads();
+    using Chain = PassChain<R, T, NT, E, T / E, +1, 0>;
+    Chain::run(b0, b1, tw, E);
+    const C2<R>* src = Chain::out ? b1 : b0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) x[e] = src[pidx(tid * E + e, E)];
+    if constexpr (Chain::out == 0) __syncthreads();   // as in fft_fwd_acc
+}
+
+// ------------------------------------------------------------------ plasma
+struct PlasmaScratch {
+    // laid out in a separate LDS area: [NT] W, [NT] c, [NT] rho (doubles), scan + reduction slots
+    double* W;
+    double* c;
+    double* rho;
+    void* scan;      // (NW+1) affine maps
+    double* red;     // 2*NW doubles
+};
+
+// rho(t) of one line from per-sample rates Wv (W(I)) and cv (avalanche
+// coefficient), chunk layout.  Newton on the chunk boundaries: thread j
+// runs its E Heun steps exactly from a guessed start value g_j (the density
+// at sample jE-1), carrying the derivative d out/d g_j; the affine block scan
+// of these linearised chunk maps gives the next guesses.  Once the guesses
+// stop moving (the start each thread used is within the tolerance of the
+// one the scan returns) the values of that pass are rho.
+template <class S, int E, int NT, class Wt, class O>
+__device__ bool plasma_scan2(const Wt (&Wv)[E], const Wt (&cv)[E], O (&rho)[E], const NLArgs& a,
+                             const PlasmaScratch& ps, int* flags)
+{
+    const int tid = threadIdx.x;
+    bool any = false;
+#pragma unroll
+    for (int e = 0; e < E; ++e) any |= Wv[e] > (Wt)0;
+    const bool line_any = __syncthreads_or(any);
+    if (!line_any && a.rho_init == 0.0) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) rho[e] = 0;
+        return false;
+    }
+    const S dt = (S)a.dt, N = (S)a.neutral, invN = (S)a.inv_neutral, invtau = (S)a.inv_recomb;
+    ps.W[tid] = (double)Wv[E - 1];
+    ps.c[tid] = (double)cv[E - 1];
+    __syncthreads();
+    // rates entering the step into sample jE come from sample jE-1; the step
+    // into n = 1 starts from E0 = 0 (ionization.cpp:25)
+    const S Wm1 = tid > 0 ? (S)ps.W[tid - 1] : (S)0, cm1 = tid > 0 ? (S)ps.c[tid - 1] : (S)0;
+    const S rho0 = (S)a.rho_init;
+    AffR<S>* sc = (AffR<S>*)ps.scan;
+    const S tol = sizeof(S) == 8 ? (S)4e-15 : (S)2e-7;
+    S g = 0;
+    bool done = false;
+    for (int it = 0; it < 24 && !done; ++it) {
+        S x = g, D = 1;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (tid == 0 && e == 0) {
+                x = rho0;
+                D = 0;
+            } else {
+                const S Wp = e == 0 ? Wm1 : ((tid == 0 && e == 1) ? (S)0 : (S)Wv[e - 1]);
+                const S cp = e == 0 ? cm1 : ((tid == 0 && e == 1) ? (S)0 : (S)cv[e - 1]);
+                S d;
+                x = heun_step(x, Wp, cp, (S)Wv[e], (S)cv[e], dt, N, invN, invtau, &d);
+                D *= d;
+            }
+            rho[e] = (O)x;
+        }
+        AffR<S> tot;
+        const AffR<S> pre = block_scan_excl<NT>(AffR<S>{D, x - D * g}, sc, &tot);
+        const S gn = tid == 0 ? (S)0 : pre.b;
+        double dm = fabs((double)(gn - g)), sm = fmax(fabs((double)gn), fabs((double)x));
+        g = gn;
+        block_max2<NT>(dm, sm, ps.red);
+        done = (dm <= (double)tol * sm) || sm == 0.0;
+    }
+    if (!done) {
+        // chunk-sequential exact pass: thread j starts from thread j-1's end value
+        if (tid == 0) *flags |= kFlagScanSlow;
+        for (int j = 0; j < NT; ++j) {
+            if (tid == j) {
+                S x = j > 0 ? (S)ps.rho[j - 1] : (S)0;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    if (j == 0 && e == 0) {
+                        x = rho0;
+                    } else {
+                        const S Wp = e == 0 ? Wm1 : ((j == 0 && e == 1) ? (S)0 : (S)Wv[e - 1]);
+                        const S cp = e == 0 ? cm1 : ((j == 0 && e == 1) ? (S)0 : (S)cv[e - 1]);
+                        S d;
+                        x = heun_step(x, Wp, cp, (S)Wv[e], (S)cv[e], dt, N, invN, invtau, &d);
+                    }
+                }
+                ps.rho[j] = (double)x;
+            }
+            __syncthreads();
+        }
+        // exact pass from the chunk starts of the sequential sweep
+        S x = tid > 0 ? (S)ps.rho[tid - 1] : (S)0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (tid == 0 && e == 0) {
+                x = rho0;
+            } else {
+                const S Wp = e == 0 ? Wm1 : ((tid == 0 && e == 1) ? (S)0 : (S)Wv[e - 1]);
+                const S cp = e == 0 ? cm1 : ((tid == 0 && e == 1) ? (S)0 : (S)cv[e - 1]);
+                S d;
+                x = heun_step(x, Wp, cp, (S)Wv[e], (S)cv[e], dt, N, invN, invtau, &d);
+            }
+            rho[e] = (O)x;
+        }
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------ NL RHS
+template <class Pol, int T, int E>
+struct LineSmem {
+    using R = typename Pol::R;
+    using SH = Shape<Pol, T, E>;
+    C2<R> b0[SH::PADN];
+    C2<R> b1[SH::PADN];
+    double pW[SH::NT], pc[SH::NT], prho[SH::NT];
+    double red[2 * SH::NW + 2];
+    AffC<double> scan[SH::NW + 1];     // large enough for every affine map type
+};
+
+template <class Pol, int T, int E>
+__device__ __forceinline__ PlasmaScratch scratch_of(LineSmem<Pol, T, E>& sm)
+{
+    return PlasmaScratch{sm.pW, sm.pc, sm.prho, (void*)sm.scan, sm.red};
+}
+
+// calculate_NLresponse of one line: x spectral (strided layout) -> out spectral (strided).
+// out = FFT(loss) + PLASMA_FUNC FFT(rho E) - i KERR FFT(((1-fR) I + R*I) E)
+// (propagator.cu:629-783), accumulated in that order.  A term whose time signal
+// is zero on the whole line (no sample above the loss threshold, rho = 0 when
+// nothing ionizes and there are no ambient carriers) has a zero spectrum and
+// its FFT is skipped.
+template <class Pol, int T, int E>
+__device__ __forceinline__ void nl_rhs(const C2<typename Pol::R> (&x)[E], C2<typename Pol::R> (&out)[E], LineSmem<Pol, T, E>& sm,
+                       const NLArgs& a, int* flags)
+{
+    using R = typename Pol::R;
+    using S = typename Pol::S;
+    using W = typename Pol::W;
+    constexpr int NT = Shape<Pol, T, E>::NT;
+    const int tid = threadIdx.x, n0 = tid * E;
+    const C2<R>* tw = (const C2<R>*)a.tw;
+
+    // E(t) = IFFT(A)/N_T, chunk layout
+    C2<R> Ev[E];
+    fft_inv<R, T, NT, E>(x, Ev, sm.b0, sm.b1, tw);
+    R lossf[E];
+    R rho[E];
+    bool anyloss = false, has_rho = false, ionizing;
+    {
+        W Ws[E];
+        bool anyw = false;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            Ev[e] = cscale(Ev[e], (R)a.inv_nt);
+            const W I = (W)Ev[e].x * (W)Ev[e].x + (W)Ev[e].y * (W)Ev[e].y;
+            Ws[e] = photoionization<W>(I, a.ionrate, flags);
+            anyw |= Ws[e] > (W)0;
+        }
+        // avalanche and loss need W > 0 somewhere (or ambient carriers)
+        ionizing = __syncthreads_or(anyw) || a.rho_init != 0.0;
+        if (ionizing) {
+            W cs[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const W I = (W)Ev[e].x * (W)Ev[e].x + (W)Ev[e].y * (W)Ev[e].y;
+                cs[e] = (W)a.sigma / ((W)a.U + (W)a.pond * I) * I;
+                const int n = n0 + e;
+                // loss[n] = -W (U + p I)/I E for 1 <= n <= T-2 above the ln-threshold (quirk, ionization.cpp:43)
+                lossf[e] = (n >= 1 && n <= T - 2 && I > (W)29.933606208922594)
+                               ? (R)(-(W)0.5 * Ws[e] * ((W)a.U + (W)a.pond * I) / I) : (R)0;
+                anyloss |= lossf[e] != (R)0;
+            }
+            has_rho = plasma_scan2<S, E, NT>(Ws, cs, rho, a, scratch_of(sm), flags);
+        }
+    }
+
+#pragma unroll
+    for (int r = 0; r < E; ++r) out[r] = C2<R>{0, 0};
+    if (ionizing && __syncthreads_or(anyloss)) fft_fwd_acc<0, R, T, NT, E>(Ev, lossf, out, sm.b0, sm.b1, tw, nullptr);
+    if (has_rho) fft_fwd_acc<1, R, T, NT, E>(Ev, rho, out, sm.b0, sm.b1, tw, a.pf);
+    if (!a.use_kerr) return;
+
+    // Raman: s_n = z s_{n-1} + I_n; y = (1-fR) I + fR M Im(s (1 - z^T) + z^{n+1} s_{T-1})
+    R yv[E];
+    if (a.use_raman) {
+        const C2<R>* zp = (const C2<R>*)a.zpow;
+        const C2<R> z = zp[1];
+        C2<R> loc = {0, 0};
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            loc = cmul(z, loc);
+            loc.x += Ev[e].x * Ev[e].x + Ev[e].y * Ev[e].y;
+        }
+        const C2<R> zE = zp[E];
+        AffC<R> tot;
+        const AffC<R> pre = block_scan_excl<NT>(AffC<R>{zE.x, zE.y, loc.x, loc.y}, (AffC<R>*)sm.scan, &tot);
+        const C2<R> slast = {tot.br, tot.bi};
+        const C2<R> omz = {(R)a.onemzT_re, (R)a.onemzT_im};
+        C2<R> sv = {pre.br, pre.bi};     // s at sample n0 - 1
+        C2<R> zq = zp[n0 + 1];           // z^{n+1} at n = n0
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const R I = Ev[e].x * Ev[e].x + Ev[e].y * Ev[e].y;
+            sv = cmul(z, sv);
+            sv.x += I;
+            const R qy = sv.x * omz.y + sv.y * omz.x + zq.x * slast.y + zq.y * slast.x;
+            yv[e] = (R)a.one_m_fr * I + (R)a.frM * qy;
+            zq = cmul(zq, z);
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) yv[e] = Ev[e].x * Ev[e].x + Ev[e].y * Ev[e].y;
+    }
+    fft_fwd_acc<2, R, T, NT, E>(Ev, yv, out, sm.b0, sm.b1, tw, a.kerr);
+}
+
+}  // namespace gs

@@ -118,6 +118,13 @@ struct gs_plan {
     gs::DevCtl* ctl_host = nullptr;
     hipGraphExec_t slot_exec = nullptr;
 
+    // reverse mode: FIELD before each recorded step (checkpoints), the step
+    // sizes, the QDHT transpose (row-major A, so the transposed-operand GEMM
+    // computes A^T x) and the cotangent buffers
+    DevBuf ckpt, adjmat, gA, gB, gT1, gT2, stage;
+    std::vector<double> adj_rk, adj_lin;
+    double last_dz_nl = 0;     // NL step size of the last accepted gs_step attempt
+
     void* buf(int id)
     {
         switch (id) {
@@ -487,6 +494,7 @@ void propagator(gs_plan* p)
     const double tol = p->s.MAX_TOLERANCE;
     for (;;) {
         double maxA2, maxNL2;
+        p->last_dz_nl = p->ZSTEP;
         nl_step(p, p->ZSTEP, &maxA2, &maxNL2);
         bool repeat = false;
         if (p->ZSTEP * p->ZSTEP * maxNL2 > tol * tol * maxA2) { repeat = true; p->STEP_N++; p->n_reject++; }
@@ -771,6 +779,8 @@ int gs_get_scalar(gs_plan* p, const char* name, double* v)
     if (n == "ZSTEP") *v = p->ZSTEP;
     else if (n == "CURRENT_Z") *v = p->CURRENT_Z;
     else if (n == "NL_SAMPLES_PER_THREAD") *v = p->nl.nl_e;
+    else if (n == "LAST_DZ_NL") *v = p->last_dz_nl;
+    else if (n == "ADJOINT_STEPS") *v = (double)p->adj_rk.size();
     else if (n == "STEP_N") *v = (double)p->STEP_N;
     else if (n == "n_Z") *v = p->n_Z;
     else if (n == "n_reject") *v = (double)p->n_reject;
@@ -1168,6 +1178,90 @@ int gs_fixed_steps(gs_plan* p, double dz, int n, double* ms4)
         for (int k = 0; k < n; ++k) p->CURRENT_Z += dz;
         p->ZSTEP = dz;
         p->STEP_N += n;
+        return 0;
+    });
+}
+
+int gs_adjoint_forward(gs_plan* p, int n, const double* dz_rk, const double* dz_lin)
+{
+    return guarded([&]() {
+        if (!p || !dz_rk || !dz_lin) return fail(GS_E_ARG, "null argument");
+        if (n <= 0) return fail(GS_E_ARG, "gs_adjoint_forward: n must be positive");
+        for (int k = 0; k < n; ++k)
+            if (!(dz_rk[k] > 0) || !(dz_lin[k] > 0)) return fail(GS_E_ARG, "gs_adjoint_forward: step sizes must be positive");
+        const size_t fb = p->field_bytes();
+        const int rp = p->rp, nr = p->s.NR, nt = p->s.NT;
+        p->adj_rk.clear();
+        p->adj_lin.clear();
+        if (p->ckpt.n != fb * (size_t)n) p->ckpt.alloc(fb * (size_t)n);
+        if (!p->adjmat.p) {
+            // row-major A = diag(m1) C diag(1/m1), padded: the GEMM reads its operand
+            // transposed, so this matrix makes it apply A^T = diag(1/m1) C^T diag(m1)
+            std::vector<double> A((size_t)rp * rp, 0.0);
+            for (int i = 0; i < nr; ++i)
+                for (int j = 0; j < nr; ++j) A[(size_t)i * rp + j] = p->s.qm1[i] * p->s.qC[(size_t)i * nr + j] / p->s.qm1[j];
+            if (p->prec == gs::kFP64) {
+                upload(p->adjmat, A.data(), A.size() * 8);
+            } else {
+                std::vector<float> f(A.begin(), A.end());
+                upload(p->adjmat, f.data(), f.size() * 4);
+            }
+            for (DevBuf* b : {&p->gA, &p->gB, &p->gT1, &p->gT2}) {
+                b->alloc(fb);
+                HCK(hipMemset(b->p, 0, fb));
+            }
+            p->stage.alloc(3 * fb);
+        }
+        const void* A = p->prec == gs::kFP64 ? p->at64.p : p->amat.p;
+        gs::LstepArgs ls = p->ls;
+        for (int k = 0; k < n; ++k) {
+            HCK(hipMemcpyAsync((char*)p->ckpt.p + (size_t)k * fb, p->field.p, fb, hipMemcpyDeviceToDevice, p->st));
+            LCK(gs::launch_nl_rk4(p->prec, nt, p->field.p, p->b1.p, dz_rk[k], p->nl, p->linemax.as<double>(), p->st));
+            ls.dz = dz_lin[k];
+            LCK(gs::launch_qdht(p->prec, rp, nt, A, p->b1.p, nullptr, &ls, p->st));
+            LCK(gs::launch_qdht(p->prec, rp, nt, A, p->fk.p, p->field.p, nullptr, p->st));
+        }
+        check_status(p, nullptr);
+        p->adj_rk.assign(dz_rk, dz_rk + n);
+        p->adj_lin.assign(dz_lin, dz_lin + n);
+        for (int k = 0; k < n; ++k) p->CURRENT_Z += dz_lin[k];
+        p->ZSTEP = dz_lin[n - 1];
+        p->STEP_N += n;
+        return 0;
+    });
+}
+
+int gs_adjoint_backward(gs_plan* p, const double* gfield, const double* gb2, int tie_b2, double* gfield0, double* gb2_0)
+{
+    return guarded([&]() {
+        if (!p || !gfield || !gfield0) return fail(GS_E_ARG, "null argument");
+        const int n = (int)p->adj_rk.size();
+        if (n == 0) return fail(GS_E_ARG, "gs_adjoint_backward: no steps recorded (call gs_adjoint_forward first)");
+        const size_t fb = p->field_bytes();
+        const int rp = p->rp, nt = p->s.NT;
+        const gs::Prec pa = p->prec == gs::kFP64 ? gs::kFP64 : gs::kFP32;   // precision of the transposed GEMMs
+        upload_field64(p, p->gA, gfield);
+        if (gb2) upload_field64(p, p->gB, gb2);
+        else HCK(hipMemsetAsync(p->gB.p, 0, fb, p->st));
+        gs::LstepArgs la = p->ls;
+        for (int k = n - 1; k >= 0; --k) {
+            // FIELD_{k+1} = Q FIELD_k(k-space): cotangent of FIELD_k(k-space)
+            LCK(gs::launch_qdht(pa, rp, nt, p->adjmat.p, p->gA.p, p->gT1.p, nullptr, p->st));
+            la.dz = p->adj_lin[k];
+            LCK(gs::launch_lstep_adj(p->prec, rp, nt, p->gT1.p, p->gB.p, p->gT2.p, la, p->st));
+            // B1(k-space) = Q B1
+            LCK(gs::launch_qdht(pa, rp, nt, p->adjmat.p, p->gT2.p, p->gT1.p, nullptr, p->st));
+            LCK(gs::launch_nl_rk4_vjp(p->prec, nt, (const char*)p->ckpt.p + (size_t)k * fb, p->gT1.p, p->gA.p,
+                                      p->stage.p, (size_t)rp * nt, p->adj_rk[k], p->nl, p->st));
+        }
+        check_status(p, nullptr);
+        if (gb2_0) download_field64(p, p->gB.p, gb2_0, p->s.NR);
+        if (tie_b2) {
+            // initial BIGBUFFER2 = QDHT(FIELD): its cotangent adds Q^T gB to FIELD's
+            LCK(gs::launch_qdht(pa, rp, nt, p->adjmat.p, p->gB.p, p->gT1.p, nullptr, p->st));
+            LCK(gs::launch_axpy(p->prec, p->gA.p, p->gT1.p, (size_t)rp * nt, p->st));
+        }
+        download_field64(p, p->gA.p, gfield0, p->s.NR);
         return 0;
     });
 }

@@ -144,6 +144,28 @@ int gs_fft_lines(gs_plan* p, const double* in, double* out, int n_lines, int sig
  * device time per step of the four kernel groups, as gs_last_timings. */
 int gs_fixed_steps(gs_plan* p, double dz, int n, double* ms4);
 
+/* Reverse mode (no counterpart in the reference, which has no adjoint).
+ *
+ * gs_adjoint_forward: n fixed steps from the plan's current state, step k with
+ * the NL RK4 at dz_rk[k] and the linear step at dz_lin[k] (the accepted attempt
+ * of propagator(), propagator.cu:125-266, runs the NL step at the previous
+ * ZSTEP and the linear step at the new one; scalar "LAST_DZ_NL" after gs_step
+ * gives the former, "ZSTEP" the latter).  FIELD is kept on the device before
+ * every step (n x N_R x N_T complex in the field precision).  The state is left
+ * at the end of the march.
+ *
+ * gs_adjoint_backward: from the cotangents of the final FIELD (gfield) and
+ * BIGBUFFER2 (gb2, NULL = 0), the cotangents of the initial FIELD (gfield0) and
+ * BIGBUFFER2 (gb2_0, may be NULL), the exact transpose of the recorded steps
+ * with the step sizes held fixed.  tie_b2 != 0 adds the BIGBUFFER2 part to
+ * gfield0 through the QDHT transpose, for runs whose initial BIGBUFFER2 is
+ * QDHT(FIELD) (as gs_plan_create sets it up).  Cotangent of a complex value z
+ * of a real objective L: dL/dRe z + i dL/dIm z.  Host arrays [N_R][N_T] complex.
+ * May be called again with other cotangents for the same recorded steps. */
+int gs_adjoint_forward(gs_plan* p, int n, const double* dz_rk, const double* dz_lin);
+int gs_adjoint_backward(gs_plan* p, const double* gfield, const double* gb2, int tie_b2, double* gfield0,
+                        double* gb2_0);
+
 /* Device time of the kernels of the last gs_fixed_step / gs_step, in ms:
  * [0] NL RK4 kernel, [1] QDHT + linear step, [2] QDHT back, [3] reductions. */
 int gs_last_timings(gs_plan* p, double* ms4);
