@@ -172,6 +172,72 @@ def batch_check(gsa, workloads, work, nruns, z_max, seed=20261202):
     return out
 
 
+def gradient_check(gsa, workloads, work, z_max, ndir, seed=20261203):
+    """Reverse mode at N_r=512 x N_t=2048 (0.5 mm / 400 fs window, fused silica,
+    1 uJ 50 fs w0=60 um): the gradient of L = sum_t |E(r=0, t)|^4 at the end of
+    a z_max march with respect to the 2048 spectral phases of the input pulse,
+    through torch.autograd (PhaseMarch: gs_adjoint_forward / _backward), on the
+    step sizes the reference controller chose (recorded once in FP64).
+    Reports the time of one gradient (forward with checkpoints + backward) per
+    precision and, along ndir seeded random phase directions, the relative
+    mismatch of each gradient's directional derivative against central
+    differences of the FP64 forward."""
+    import numpy as np
+    import torch
+    from gsolver_amd.adjoint import PhaseMarch, record_steps
+    cfg = workloads.write_config(work, "c0", "c3", N_R=512, N_T=2048, N_Z=2, Z_MAX=z_max)
+    rec = gsa.Plan(cfg, precision="fp64")
+    t0 = time.perf_counter()
+    dz_rk, dz_lin = record_steps(rec, z_end=z_max)
+    t_rec = time.perf_counter() - t0
+    field0 = rec.get()
+    rec.close()
+    n = len(dz_rk)
+    print("[bench] gradient: %d recorded steps to z=%g m (%.1f s)" % (n, z_max, t_rec), file=sys.stderr, flush=True)
+
+    def objective(field):
+        return (torch.fft.ifft(field[0]).abs() ** 4).sum()
+
+    out = {"workload": "N_r=512 x N_t=2048, 0.5 mm, 400 fs, fused silica, 1 uJ 50 fs w0=60 um, %g m" % z_max,
+           "params": int(field0.shape[1]), "steps": n, "objective": "sum_t |E(0,t)|^4", "directions": ndir}
+    grads, plans = {}, {}
+    for prec in ("mixed", "fp32", "fp64"):
+        p = gsa.Plan(cfg, precision=prec)
+        plans[prec] = p
+        phi = torch.zeros(p.T, dtype=torch.float64, requires_grad=True)
+        t0 = time.perf_counter()
+        L = objective(PhaseMarch.apply(phi, p, field0, dz_rk, dz_lin))
+        t1 = time.perf_counter()
+        L.backward()
+        t2 = time.perf_counter()
+        grads[prec] = phi.grad.numpy().copy()
+        out[prec] = {"forward_s": t1 - t0, "backward_s": t2 - t1, "gradient_s": t2 - t0,
+                     "backward_over_forward": (t2 - t1) / (t1 - t0)}
+        if prec != "fp64":
+            p.close()      # frees its checkpoints (steps x N_r x N_t complex)
+        print("[bench] gradient %s: forward %.3f s, backward %.3f s" % (prec, t1 - t0, t2 - t1),
+              file=sys.stderr, flush=True)
+    rng = np.random.default_rng(seed)
+    p64 = plans["fp64"]
+    eps = 1e-5
+    errs = {prec: [] for prec in grads}
+    with torch.no_grad():
+        for _ in range(ndir):
+            v = rng.standard_normal(p64.T)
+            lp = objective(PhaseMarch.apply(torch.tensor(eps * v), p64, field0, dz_rk, dz_lin)).item()
+            lm = objective(PhaseMarch.apply(torch.tensor(-eps * v), p64, field0, dz_rk, dz_lin)).item()
+            d = (lp - lm) / (2 * eps)
+            for prec, g in grads.items():
+                errs[prec].append(abs(float(g @ v) - d) / abs(d))
+    for prec in grads:
+        out[prec]["fd_rel_err_max"] = float(np.max(errs[prec]))
+        out[prec]["fd_rel_err_median"] = float(np.median(errs[prec]))
+    p64.close()
+    print("[bench] gradient vs central differences (max over %d directions): %s" % (
+        ndir, {k: "%.2e" % out[k]["fd_rel_err_max"] for k in grads}), file=sys.stderr, flush=True)
+    return out
+
+
 def roofline(kind, ms, R, T, prec, kernel=None):
     if kind == "qdht":
         flops = 4.0 * R * R * T                 # one transform: real R x R times 2T real columns
@@ -200,6 +266,9 @@ def main():
     ap.add_argument("--batch-zmax", type=float, default=5e-6, help="distance of each scan run in m")
     ap.add_argument("--accuracy-zmax", type=float, default=1e-3,
                     help="distance of the accuracy-check march in m (two dump planes)")
+    ap.add_argument("--gradient-zmax", type=float, default=1e-2,
+                    help="march length of the reverse-mode check at N_r=512 x N_t=2048 (0: skip)")
+    ap.add_argument("--gradient-dirs", type=int, default=32, help="random directions of the gradient check")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -255,6 +324,8 @@ def main():
 
     batch = batch_check(gsa, workloads, work, args.batch_runs, args.batch_zmax) if args.batch_runs > 0 else None
 
+    grad = gradient_check(gsa, workloads, work, args.gradient_zmax, args.gradient_dirs) if args.gradient_zmax > 0 else None
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -280,6 +351,7 @@ def main():
         "field_rel_l2_vs_fp64": err or None,
         "accuracy_march_vs_fp64": acc,
         "energy_width_scan": batch,
+        "phase_gradient": grad,
         "kernel_ms_per_step": {p: res[p]["kernel_ms"] for p in res},
         "roofline": roof,
         "roofline_qdht": roofline("qdht", km["qdht_back"], R, T, "mixed", "k_qdht_bf16x3_t128<false>"),

@@ -18,13 +18,16 @@ import torch
 from . import B2, FIELD
 
 
-def record_steps(plan, n_steps):
-    """Advance `plan` by n_steps accepted steps of the reference controller
-    (gs_step) and return the (dz_rk, dz_lin) arrays that replay them: the NL
-    step of an accepted attempt runs at the previous ZSTEP, the linear step at
-    the new one."""
+def record_steps(plan, n_steps=None, z_end=None):
+    """Advance `plan` with the reference controller (gs_step) by n_steps
+    accepted steps, or until CURRENT_Z reaches z_end (a ZNET plane: the last
+    step is clamped onto it), and return the (dz_rk, dz_lin) arrays that replay
+    them: the NL step of an accepted attempt runs at the previous ZSTEP, the
+    linear step at the new one."""
+    if (n_steps is None) == (z_end is None):
+        raise ValueError("give exactly one of n_steps and z_end")
     rk, lin = [], []
-    for _ in range(int(n_steps)):
+    while (len(rk) < n_steps) if n_steps is not None else (plan.scalar("CURRENT_Z") < z_end * (1 - 1e-12)):
         plan.step()
         rk.append(plan.scalar("LAST_DZ_NL"))
         lin.append(plan.scalar("ZSTEP"))

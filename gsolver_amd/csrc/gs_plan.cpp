@@ -118,11 +118,16 @@ struct gs_plan {
     gs::DevCtl* ctl_host = nullptr;
     hipGraphExec_t slot_exec = nullptr;
 
-    // reverse mode: FIELD before each recorded step (checkpoints), the step
-    // sizes, the QDHT transpose (row-major A, so the transposed-operand GEMM
-    // computes A^T x) and the cotangent buffers
-    DevBuf ckpt, adjmat, gA, gB, gT1, gT2, stage;
+    // reverse mode: FIELD before each step of the segment being reversed
+    // (ckpt), FIELD and BIGBUFFER2 at the start of every segment (snaps), the
+    // replay buffers that recompute a segment, the step sizes, the QDHT
+    // transpose (row-major A, so the transposed-operand GEMM computes A^T x)
+    // and the cotangent buffers.  A march whose per-step checkpoints fit in
+    // adj_budget bytes is one segment (no recomputation).
+    DevBuf ckpt, snaps, rfield, rb1, rb2, rfk, adjmat, gA, gB, gT1, gT2, stage;
     std::vector<double> adj_rk, adj_lin;
+    int adj_seg = 0;                       // steps per segment
+    double adj_budget = 16.0 * (1 << 30);  // bytes of per-step checkpoints kept without recomputation
     double last_dz_nl = 0;     // NL step size of the last accepted gs_step attempt
 
     void* buf(int id)
@@ -781,6 +786,8 @@ int gs_get_scalar(gs_plan* p, const char* name, double* v)
     else if (n == "NL_SAMPLES_PER_THREAD") *v = p->nl.nl_e;
     else if (n == "LAST_DZ_NL") *v = p->last_dz_nl;
     else if (n == "ADJOINT_STEPS") *v = (double)p->adj_rk.size();
+    else if (n == "ADJOINT_SEGMENT") *v = p->adj_seg;
+    else if (n == "ADJOINT_CHECKPOINT_BYTES") *v = p->adj_budget;
     else if (n == "STEP_N") *v = (double)p->STEP_N;
     else if (n == "n_Z") *v = p->n_Z;
     else if (n == "n_reject") *v = (double)p->n_reject;
@@ -814,6 +821,10 @@ int gs_set_scalar(gs_plan* p, const char* name, double v)
     else if (n == "CURRENT_Z") p->CURRENT_Z = v;
     else if (n == "n_Z") p->n_Z = (int)v;
     else if (n == "MAX_TOLERANCE") p->s.MAX_TOLERANCE = v;
+    else if (n == "ADJOINT_CHECKPOINT_BYTES") {
+        if (!(v >= 0)) return fail(GS_E_ARG, "ADJOINT_CHECKPOINT_BYTES must be >= 0");
+        p->adj_budget = v;
+    }
     else return fail(GS_E_ARG, "scalar " + n + " is not settable");
     return 0;
 }
@@ -1193,7 +1204,23 @@ int gs_adjoint_forward(gs_plan* p, int n, const double* dz_rk, const double* dz_
         const int rp = p->rp, nr = p->s.NR, nt = p->s.NT;
         p->adj_rk.clear();
         p->adj_lin.clear();
-        if (p->ckpt.n != fb * (size_t)n) p->ckpt.alloc(fb * (size_t)n);
+        // one segment when every step's FIELD fits the budget; otherwise segments
+        // of K ~ sqrt(2n) steps (memory ceil(n/K) x 2 + K fields, one recomputation)
+        int K = n;
+        if ((double)n * (double)fb > p->adj_budget) K = std::max(1, (int)std::ceil(std::sqrt(2.0 * n)));
+        const int nseg = (n + K - 1) / K;
+        if (p->ckpt.n != fb * (size_t)K) p->ckpt.alloc(fb * (size_t)K);
+        if (K < n) {
+            if (p->snaps.n != 2 * fb * (size_t)nseg) p->snaps.alloc(2 * fb * (size_t)nseg);
+            for (DevBuf* b : {&p->rfield, &p->rb1, &p->rb2, &p->rfk})
+                if (b->n != fb) {
+                    b->alloc(fb);
+                    HCK(hipMemset(b->p, 0, fb));
+                }
+        } else {
+            p->snaps.alloc(0);
+        }
+        p->adj_seg = K;
         if (!p->adjmat.p) {
             // row-major A = diag(m1) C diag(1/m1), padded: the GEMM reads its operand
             // transposed, so this matrix makes it apply A^T = diag(1/m1) C^T diag(m1)
@@ -1215,7 +1242,13 @@ int gs_adjoint_forward(gs_plan* p, int n, const double* dz_rk, const double* dz_
         const void* A = p->prec == gs::kFP64 ? p->at64.p : p->amat.p;
         gs::LstepArgs ls = p->ls;
         for (int k = 0; k < n; ++k) {
-            HCK(hipMemcpyAsync((char*)p->ckpt.p + (size_t)k * fb, p->field.p, fb, hipMemcpyDeviceToDevice, p->st));
+            if (K == n) {
+                HCK(hipMemcpyAsync((char*)p->ckpt.p + (size_t)k * fb, p->field.p, fb, hipMemcpyDeviceToDevice, p->st));
+            } else if (k % K == 0) {
+                char* sn = (char*)p->snaps.p + (size_t)(k / K) * 2 * fb;
+                HCK(hipMemcpyAsync(sn, p->field.p, fb, hipMemcpyDeviceToDevice, p->st));
+                HCK(hipMemcpyAsync(sn + fb, p->b2.p, fb, hipMemcpyDeviceToDevice, p->st));
+            }
             LCK(gs::launch_nl_rk4(p->prec, nt, p->field.p, p->b1.p, dz_rk[k], p->nl, p->linemax.as<double>(), p->st));
             ls.dz = dz_lin[k];
             LCK(gs::launch_qdht(p->prec, rp, nt, A, p->b1.p, nullptr, &ls, p->st));
@@ -1238,21 +1271,43 @@ int gs_adjoint_backward(gs_plan* p, const double* gfield, const double* gb2, int
         const int n = (int)p->adj_rk.size();
         if (n == 0) return fail(GS_E_ARG, "gs_adjoint_backward: no steps recorded (call gs_adjoint_forward first)");
         const size_t fb = p->field_bytes();
-        const int rp = p->rp, nt = p->s.NT;
+        const int rp = p->rp, nt = p->s.NT, K = p->adj_seg;
         const gs::Prec pa = p->prec == gs::kFP64 ? gs::kFP64 : gs::kFP32;   // precision of the transposed GEMMs
+        const void* A = p->prec == gs::kFP64 ? p->at64.p : p->amat.p;
         upload_field64(p, p->gA, gfield);
         if (gb2) upload_field64(p, p->gB, gb2);
         else HCK(hipMemsetAsync(p->gB.p, 0, fb, p->st));
         gs::LstepArgs la = p->ls;
-        for (int k = n - 1; k >= 0; --k) {
-            // FIELD_{k+1} = Q FIELD_k(k-space): cotangent of FIELD_k(k-space)
-            LCK(gs::launch_qdht(pa, rp, nt, p->adjmat.p, p->gA.p, p->gT1.p, nullptr, p->st));
-            la.dz = p->adj_lin[k];
-            LCK(gs::launch_lstep_adj(p->prec, rp, nt, p->gT1.p, p->gB.p, p->gT2.p, la, p->st));
-            // B1(k-space) = Q B1
-            LCK(gs::launch_qdht(pa, rp, nt, p->adjmat.p, p->gT2.p, p->gT1.p, nullptr, p->st));
-            LCK(gs::launch_nl_rk4_vjp(p->prec, nt, (const char*)p->ckpt.p + (size_t)k * fb, p->gT1.p, p->gA.p,
-                                      p->stage.p, (size_t)rp * nt, p->adj_rk[k], p->nl, p->st));
+        gs::LstepArgs lr = p->ls;        // the replay's linear step writes the replay buffers
+        lr.b2 = p->rb2.p;
+        lr.fk = p->rfk.p;
+        for (int k1 = n; k1 > 0;) {
+            const int k0 = (k1 - 1) / K * K;
+            if (K < n) {
+                // recompute the segment's FIELDs from its snapshot (same kernels, same order)
+                const char* sn = (const char*)p->snaps.p + (size_t)(k0 / K) * 2 * fb;
+                HCK(hipMemcpyAsync(p->rfield.p, sn, fb, hipMemcpyDeviceToDevice, p->st));
+                HCK(hipMemcpyAsync(p->rb2.p, sn + fb, fb, hipMemcpyDeviceToDevice, p->st));
+                for (int k = k0; k < k1; ++k) {
+                    HCK(hipMemcpyAsync((char*)p->ckpt.p + (size_t)(k - k0) * fb, p->rfield.p, fb, hipMemcpyDeviceToDevice, p->st));
+                    if (k + 1 == k1) break;    // the segment's last FIELD is all the reverse sweep needs
+                    LCK(gs::launch_nl_rk4(p->prec, nt, p->rfield.p, p->rb1.p, p->adj_rk[k], p->nl, p->linemax.as<double>(), p->st));
+                    lr.dz = p->adj_lin[k];
+                    LCK(gs::launch_qdht(p->prec, rp, nt, A, p->rb1.p, nullptr, &lr, p->st));
+                    LCK(gs::launch_qdht(p->prec, rp, nt, A, p->rfk.p, p->rfield.p, nullptr, p->st));
+                }
+            }
+            for (int k = k1 - 1; k >= k0; --k) {
+                // FIELD_{k+1} = Q FIELD_k(k-space): cotangent of FIELD_k(k-space)
+                LCK(gs::launch_qdht(pa, rp, nt, p->adjmat.p, p->gA.p, p->gT1.p, nullptr, p->st));
+                la.dz = p->adj_lin[k];
+                LCK(gs::launch_lstep_adj(p->prec, rp, nt, p->gT1.p, p->gB.p, p->gT2.p, la, p->st));
+                // B1(k-space) = Q B1
+                LCK(gs::launch_qdht(pa, rp, nt, p->adjmat.p, p->gT2.p, p->gT1.p, nullptr, p->st));
+                LCK(gs::launch_nl_rk4_vjp(p->prec, nt, (const char*)p->ckpt.p + (size_t)(k - k0) * fb, p->gT1.p, p->gA.p,
+                                          p->stage.p, (size_t)rp * nt, p->adj_rk[k], p->nl, p->st));
+            }
+            k1 = k0;
         }
         check_status(p, nullptr);
         if (gb2_0) download_field64(p, p->gB.p, gb2_0, p->s.NR);

@@ -150,9 +150,12 @@ int gs_fixed_steps(gs_plan* p, double dz, int n, double* ms4);
  * the NL RK4 at dz_rk[k] and the linear step at dz_lin[k] (the accepted attempt
  * of propagator(), propagator.cu:125-266, runs the NL step at the previous
  * ZSTEP and the linear step at the new one; scalar "LAST_DZ_NL" after gs_step
- * gives the former, "ZSTEP" the latter).  FIELD is kept on the device before
- * every step (n x N_R x N_T complex in the field precision).  The state is left
- * at the end of the march.
+ * gives the former, "ZSTEP" the latter).  The state is left at the end of the
+ * march.  FIELD is kept on the device before every step when n of them fit in
+ * the scalar "ADJOINT_CHECKPOINT_BYTES" (default 16 GiB; gs_set_scalar);
+ * otherwise FIELD and BIGBUFFER2 are kept every K = ceil(sqrt(2n)) steps and
+ * gs_adjoint_backward recomputes one segment of K steps at a time (memory
+ * (2 ceil(n/K) + K) fields, one extra forward).
  *
  * gs_adjoint_backward: from the cotangents of the final FIELD (gfield) and
  * BIGBUFFER2 (gb2, NULL = 0), the cotangents of the initial FIELD (gfield0) and

@@ -50,6 +50,30 @@ def test_adjoint_march_vs_oracle(tmp_path, prec, nr):
     assert rel_l2(g0b, g0r2) < tol and rel_l2(b0b, b0r2) < tol
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_adjoint_segmented_matches_single_segment(tmp_path, prec):
+    """A checkpoint budget below one field per step makes the backward recompute
+    segments (7 steps: K = 4, segments of 4 and 3) from FIELD / BIGBUFFER2
+    snapshots; the replay runs the same kernels in the same order, so the
+    cotangents are identical to the one-segment run."""
+    cfg = write_cfg(tmp_path / "c.txt", **HOT)
+    rk = np.array([2e-6, 1.5e-6, 2.5e-6, 1e-6, 2e-6, 1.8e-6, 2.2e-6])
+    lin = np.roll(rk, -1)
+    rng = np.random.default_rng(11)
+    gF, gB = crand(rng, (64, 512)), crand(rng, (64, 512))
+    out = []
+    for budget in (None, 1.0):
+        p = gsa.Plan(cfg, precision=prec)
+        if budget is not None:
+            p.set_scalar("ADJOINT_CHECKPOINT_BYTES", budget)
+        p.adjoint_forward(rk, lin)
+        out.append((p.scalar("ADJOINT_SEGMENT"), p.get()) + p.adjoint_backward(gF, gB, tie_b2=True))
+        p.close()
+    assert out[0][0] == 7 and out[1][0] == 4
+    for a, b in zip(out[0][1:], out[1][1:]):
+        assert np.array_equal(a, b)
+
+
 def test_adjoint_tied_b2(tmp_path):
     """tie_b2 folds the initial BIGBUFFER2 = QDHT(FIELD) cotangent into FIELD's:
     gfield0(tied) = gfield0 + Q^T gb2_0, Q = diag(m1) C diag(1/m1)."""
