@@ -172,12 +172,13 @@ def batch_check(gsa, workloads, work, nruns, z_max, seed=20261202):
     return out
 
 
-def gradient_check(gsa, workloads, work, z_max, ndir, seed=20261203):
-    """Reverse mode at N_r=512 x N_t=2048 (0.5 mm / 400 fs window, fused silica,
-    1 uJ 50 fs w0=60 um): the gradient of L = sum_t |E(r=0, t)|^4 at the end of
-    a z_max march with respect to the 2048 spectral phases of the input pulse,
-    through torch.autograd (PhaseMarch: gs_adjoint_forward / _backward), on the
-    step sizes the reference controller chose (recorded once in FP64).
+def gradient_check(gsa, workloads, work, z_max, ndir, energy, max_steps, seed=20261203):
+    """Reverse mode at N_r=512 x N_t=2048 (0.5 mm / 400 fs window, fused
+    silica, 50 fs w0=60 um pulse of the given energy): the gradient of
+    L = sum_t |E(r=0, t)|^4 at the end of a z_max march with respect to the
+    2048 spectral phases of the input pulse, through torch.autograd
+    (PhaseMarch: gs_adjoint_forward / _backward), on the step sizes the
+    reference controller chose (recorded once in FP64; at most max_steps).
     Reports the time of one gradient (forward with checkpoints + backward) per
     precision and, along ndir seeded random phase directions, the relative
     mismatch of each gradient's directional derivative against central
@@ -185,21 +186,26 @@ def gradient_check(gsa, workloads, work, z_max, ndir, seed=20261203):
     import numpy as np
     import torch
     from gsolver_amd.adjoint import PhaseMarch, record_steps
-    cfg = workloads.write_config(work, "c0", "c3", N_R=512, N_T=2048, N_Z=2, Z_MAX=z_max)
+    cfg = workloads.write_config(work, "c0", "c3", N_R=512, N_T=2048, N_Z=2, Z_MAX=z_max, ENERGY="%g" % energy)
     rec = gsa.Plan(cfg, precision="fp64")
     t0 = time.perf_counter()
-    dz_rk, dz_lin = record_steps(rec, z_end=z_max)
+    dz_rk, dz_lin = record_steps(rec, z_end=z_max, max_steps=max_steps)
     t_rec = time.perf_counter() - t0
-    field0 = rec.get()
+    z_end = rec.scalar("CURRENT_Z")
     rec.close()
+    p0 = gsa.Plan(cfg, precision="fp64")     # the config's initial field (rec is at z_end)
+    field0 = p0.get()
+    p0.close()
     n = len(dz_rk)
-    print("[bench] gradient: %d recorded steps to z=%g m (%.1f s)" % (n, z_max, t_rec), file=sys.stderr, flush=True)
+    print("[bench] gradient: %d recorded steps to z=%g m (%.1f s)" % (n, z_end, t_rec), file=sys.stderr, flush=True)
 
     def objective(field):
         return (torch.fft.ifft(field[0]).abs() ** 4).sum()
 
-    out = {"workload": "N_r=512 x N_t=2048, 0.5 mm, 400 fs, fused silica, 1 uJ 50 fs w0=60 um, %g m" % z_max,
-           "params": int(field0.shape[1]), "steps": n, "objective": "sum_t |E(0,t)|^4", "directions": ndir}
+    out = {"workload": "N_r=512 x N_t=2048, 0.5 mm, 400 fs, fused silica, %g uJ 50 fs w0=60 um, z %g m"
+                       % (energy * 1e6, z_end),
+           "params": int(field0.shape[1]), "steps": n, "z_m": z_end, "objective": "sum_t |E(0,t)|^4",
+           "directions": ndir}
     grads, plans = {}, {}
     for prec in ("mixed", "fp32", "fp64"):
         p = gsa.Plan(cfg, precision=prec)
@@ -222,7 +228,9 @@ def gradient_check(gsa, workloads, work, z_max, ndir, seed=20261203):
     eps = 1e-5
     errs = {prec: [] for prec in grads}
     with torch.no_grad():
-        for _ in range(ndir):
+        for j in range(ndir):
+            if j % 8 == 0:
+                print("[bench] gradient: direction %d of %d" % (j, ndir), file=sys.stderr, flush=True)
             v = rng.standard_normal(p64.T)
             lp = objective(PhaseMarch.apply(torch.tensor(eps * v), p64, field0, dz_rk, dz_lin)).item()
             lm = objective(PhaseMarch.apply(torch.tensor(-eps * v), p64, field0, dz_rk, dz_lin)).item()
@@ -268,7 +276,12 @@ def main():
                     help="distance of the accuracy-check march in m (two dump planes)")
     ap.add_argument("--gradient-zmax", type=float, default=1e-2,
                     help="march length of the reverse-mode check at N_r=512 x N_t=2048 (0: skip)")
-    ap.add_argument("--gradient-dirs", type=int, default=32, help="random directions of the gradient check")
+    ap.add_argument("--gradient-dirs", type=int, default=8,
+                    help="random directions of the gradient check (each costs two FP64 marches)")
+    ap.add_argument("--gradient-energy", type=float, default=0.3e-6,
+                    help="pulse energy of the gradient check in J (0.3 uJ: about twice the critical power, "
+                         "collapse beyond 1 cm)")
+    ap.add_argument("--gradient-max-steps", type=int, default=50000, help="cap on the recorded steps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -324,7 +337,8 @@ def main():
 
     batch = batch_check(gsa, workloads, work, args.batch_runs, args.batch_zmax) if args.batch_runs > 0 else None
 
-    grad = gradient_check(gsa, workloads, work, args.gradient_zmax, args.gradient_dirs) if args.gradient_zmax > 0 else None
+    grad = (gradient_check(gsa, workloads, work, args.gradient_zmax, args.gradient_dirs, args.gradient_energy,
+                           args.gradient_max_steps) if args.gradient_zmax > 0 else None)
 
     cpu = None
     if not args.no_cpu_baseline:

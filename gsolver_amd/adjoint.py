@@ -18,16 +18,18 @@ import torch
 from . import B2, FIELD
 
 
-def record_steps(plan, n_steps=None, z_end=None):
+def record_steps(plan, n_steps=None, z_end=None, max_steps=None):
     """Advance `plan` with the reference controller (gs_step) by n_steps
     accepted steps, or until CURRENT_Z reaches z_end (a ZNET plane: the last
-    step is clamped onto it), and return the (dz_rk, dz_lin) arrays that replay
-    them: the NL step of an accepted attempt runs at the previous ZSTEP, the
-    linear step at the new one."""
+    step is clamped onto it) or max_steps steps were taken, and return the
+    (dz_rk, dz_lin) arrays that replay them: the NL step of an accepted attempt
+    runs at the previous ZSTEP, the linear step at the new one."""
     if (n_steps is None) == (z_end is None):
         raise ValueError("give exactly one of n_steps and z_end")
     rk, lin = [], []
-    while (len(rk) < n_steps) if n_steps is not None else (plan.scalar("CURRENT_Z") < z_end * (1 - 1e-12)):
+    cap = max_steps if max_steps is not None else float("inf")
+    while ((len(rk) < n_steps) if n_steps is not None
+           else (plan.scalar("CURRENT_Z") < z_end * (1 - 1e-12) and len(rk) < cap)):
         plan.step()
         rk.append(plan.scalar("LAST_DZ_NL"))
         lin.append(plan.scalar("ZSTEP"))
