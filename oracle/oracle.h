@@ -23,6 +23,8 @@
  * mpmath golden vectors (tests/golden/make_special.py) for Bessel J0/J1/zeros, Dawson and the
  * complete elliptic integrals, brute-force DFT for the FFT, closed forms
  * for the Raman convolution, the plasma recurrence and the RK4 combine.
+ * The reverse mode of o_adjoint.c (no reference counterpart) is pinned by
+ * central finite differences of this forward restatement.
  * Parity unpinned (no reference-held vector exists): the Keldysh rate table
  * values, and the nonlinear march end to end.
  *
