@@ -10,10 +10,11 @@ namespace gs {
 // FP32 field lines keep the RK4 input row A and the k1 + 2k2 + 2k3 + k4
 // accumulator in LDS (2 x 32 KiB at N_T = 4096, after LineSmem); FP64 lines
 // do not fit and use the global rows.
-template <class Pol, int T>
+template <class Pol, int T, int E>
 struct RK4Rows {
-    static constexpr bool in_lds = sizeof(typename Pol::R) == 4;
-    static constexpr size_t bytes = in_lds ? 2 * (size_t)T * sizeof(C2<typename Pol::R>) : 0;
+    static constexpr size_t row_bytes = 2 * (size_t)T * sizeof(C2<typename Pol::R>);
+    static constexpr bool in_lds = sizeof(typename Pol::R) == 4 && sizeof(LineSmem<Pol, T, E>) + row_bytes <= 160 * 1024;
+    static constexpr size_t bytes = in_lds ? row_bytes : 0;
 };
 
 template <class Pol, int T, int E>
@@ -23,7 +24,7 @@ __global__ __launch_bounds__((Shape<Pol, T, E>::NT), GS_NL_WAVES) void k_nl_rk4(
 {
     using R = typename Pol::R;
     constexpr int NT = Shape<Pol, T, E>::NT;
-    constexpr bool ROWS_LDS = RK4Rows<Pol, T>::in_lds;
+    constexpr bool ROWS_LDS = RK4Rows<Pol, T, E>::in_lds;
     if (a.ctl) {
         if (a.ctl->state != 0) return;
         dz = a.ctl->dz;
@@ -226,7 +227,7 @@ template <class Pol, int T, int E>
 static int nl_rk4_TE(const void* field, void* b1, double dz, const NLArgs& a, double* linemax, hipStream_t st)
 {
     using R = typename Pol::R;
-    const size_t sm = sizeof(LineSmem<Pol, T, E>) + RK4Rows<Pol, T>::bytes;
+    const size_t sm = sizeof(LineSmem<Pol, T, E>) + RK4Rows<Pol, T, E>::bytes;
     auto kf = k_nl_rk4<Pol, T, E>;
     HIPCHK(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
     hipLaunchKernelGGL(kf, dim3(a.nr), dim3(Shape<Pol, T, E>::NT), sm, st, (const C2<R>*)field, (C2<R>*)b1, dz, a, linemax);
@@ -234,11 +235,12 @@ static int nl_rk4_TE(const void* field, void* b1, double dz, const NLArgs& a, do
     return 0;
 }
 
-// samples per thread: the default, or 8/16 when a.nl_e asks for it (FP32/mixed, N_T >= 1024)
+// samples per thread: the default, or 4/8/16 when a.nl_e asks for it (FP32/mixed, N_T >= 1024)
 template <class Pol, int T>
 static int nl_rk4_T(const void* field, void* b1, double dz, const NLArgs& a, double* linemax, hipStream_t st)
 {
     if constexpr (sizeof(typename Pol::R) == 4 && T >= 1024) {
+        if (a.nl_e == 4) return nl_rk4_TE<Pol, T, 4>(field, b1, dz, a, linemax, st);
         if (a.nl_e == 8) return nl_rk4_TE<Pol, T, 8>(field, b1, dz, a, linemax, st);
         if (a.nl_e == 16) return nl_rk4_TE<Pol, T, 16>(field, b1, dz, a, linemax, st);
     }

@@ -405,10 +405,11 @@ void build_plan(gs_plan* p)
 }
 
 // ---- autotuner of the line kernel's samples per thread
-// For FP32 / mixed lines of N_T >= 1024 the RK4 kernel has two shapes (8 or 16
-// samples per thread: 512 or 256 threads per line, different register and LDS
-// footprints).  The first plan of a (precision, N_T, N_R) in the process times
-// both on its initial field and keeps the faster; GS_NL_E overrides.
+// For FP32 / mixed lines of N_T >= 1024 the RK4 kernel has three shapes (4, 8
+// or 16 samples per thread: N_T/4, N_T/8 or N_T/16 threads per line, different
+// register and LDS footprints).  The first plan of a (precision, N_T, N_R) in
+// the process times all of them on its initial field and keeps the fastest
+// when it beats 8 by 5 %; GS_NL_E overrides.
 void autotune_nl(gs_plan* p)
 {
     static std::mutex mu;
@@ -429,10 +430,10 @@ void autotune_nl(gs_plan* p)
     HCK(hipEventCreate(&e1));
     // candidates alternate (warm-up round first), each keeps its best time; the
     // default shape (8) is replaced only by a clear (5 %) win
-    const int cand[2] = {8, 16};
-    float tmin[2] = {1e30f, 1e30f};
+    const int cand[3] = {8, 16, 4};
+    float tmin[3] = {1e30f, 1e30f, 1e30f};
     for (int rep = 0; rep < 6; ++rep)
-        for (int c = 0; c < 2; ++c) {
+        for (int c = 0; c < 3; ++c) {
             a.nl_e = cand[c];
             HCK(hipEventRecord(e0, p->st));
             LCK(gs::launch_nl_rk4(p->prec, p->s.NT, p->field.p, p->b1.p, p->ZSTEP, a, p->linemax.as<double>(), p->st));
@@ -442,7 +443,13 @@ void autotune_nl(gs_plan* p)
             HCK(hipEventElapsedTime(&t, e0, e1));
             if (rep > 0) tmin[c] = std::min(tmin[c], t);
         }
-    const int best = tmin[1] < 0.95f * tmin[0] ? cand[1] : cand[0];
+    int best = cand[0];
+    float tb = 0.95f * tmin[0];
+    for (int c = 1; c < 3; ++c)
+        if (tmin[c] < tb) {
+            tb = tmin[c];
+            best = cand[c];
+        }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     HCK(hipMemsetAsync(p->status.p, 0, sizeof(gs::DevStatus), p->st));

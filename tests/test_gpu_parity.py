@@ -116,6 +116,27 @@ def test_nl_step_vs_oracle(tmp_path, prec):
     assert abs(b / b_ref - 1) < {"fp64": 1e-11, "fp32": 1e-4, "mixed": 1e-4}[prec]
 
 
+@pytest.mark.parametrize("nl_e", [4, 8, 16])
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+def test_line_kernel_shapes_vs_oracle(tmp_path, monkeypatch, prec, nl_e):
+    """Every samples-per-thread shape the autotuner may pick (4, 8, 16 at
+    N_T >= 1024) gives the oracle's NL step at N_T = 2048 over a 400 fs
+    window, on a field that ionizes with its peak (about 20 in solver units)
+    below the loss gate I > 29.93.  (Over a 200 fs window the 2048-point
+    spectrum reaches far beyond the pulse and FP32 itself, the FP32 oracle
+    included, departs from FP64 by ~1e-4.)"""
+    monkeypatch.setenv("GS_NL_E", str(nl_e))
+    p, o = pair(tmp_path, prec, **dict(HOT, N_T="2048", ENERGY="12e-6", T_MIN="-200e-15", T_MAX="200e-15"))
+    assert p.scalar("NL_SAMPLES_PER_THREAD") == nl_e
+    dz = 2e-6
+    o.set_scalar("ZSTEP", dz)
+    o.nl_step()
+    p.nl_step(dz)
+    err = rel_l2(p.get(gsa.B1), o.get("b1"))
+    print("shape", prec, nl_e, err)
+    assert err < 2e-5
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
 @pytest.mark.parametrize("nr", [64, 100])
 def test_fixed_steps_vs_oracle(tmp_path, prec, nr):
