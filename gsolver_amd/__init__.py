@@ -197,8 +197,13 @@ class Plan:
         _check(lib().gs_fixed_step(self.h, dz, C.byref(a), C.byref(b)))
         return a.value, b.value
 
-    def fixed_steps(self, dz, n):
-        """n fixed-dz steps with one synchronisation; returns the mean kernel times per step."""
+    def fixed_steps(self, dz, n, timings=True):
+        """n fixed-dz steps with one synchronisation; returns the mean kernel
+        times per step (timings=False: no timing events between the kernels,
+        returns None)."""
+        if not timings:
+            _check(lib().gs_fixed_steps(self.h, dz, int(n), None))
+            return None
         t = np.zeros(4)
         _check(lib().gs_fixed_steps(self.h, dz, int(n), _d(t)))
         return dict(zip(("nl_rk4", "qdht_lstep", "qdht_back", "reduce"), t))

@@ -1159,14 +1159,28 @@ int gs_fixed_steps(gs_plan* p, double dz, int n, double* ms4)
     return guarded([&]() {
         if (!p) return fail(GS_E_ARG, "null plan");
         if (n <= 0) return fail(GS_E_ARG, "gs_fixed_steps: n must be positive");
+        const void* A = p->prec == gs::kFP64 ? p->at64.p : p->amat.p;
+        p->ls.dz = dz;
+        if (!ms4) {
+            // no timings asked for: the kernels alone, no event markers between them
+            for (int k = 0; k < n; ++k) {
+                LCK(gs::launch_nl_rk4(p->prec, p->s.NT, p->field.p, p->b1.p, dz, p->nl, p->linemax.as<double>(), p->st));
+                LCK(gs::launch_reduce_max(p->s.NR, p->linemax.as<double>(), p->status.as<gs::DevStatus>(), p->st));
+                LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->b1.p, nullptr, &p->ls, p->st));
+                LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->fk.p, p->field.p, nullptr, p->st));
+            }
+            check_status(p, nullptr);
+            for (int k = 0; k < n; ++k) p->CURRENT_Z += dz;
+            p->ZSTEP = dz;
+            p->STEP_N += n;
+            return 0;
+        }
         std::vector<hipEvent_t> ev((size_t)5 * n, nullptr);
         struct EvGuard {
             std::vector<hipEvent_t>& v;
             ~EvGuard() { for (auto e : v) if (e) (void)hipEventDestroy(e); }
         } guard{ev};
         for (auto& e : ev) HCK(hipEventCreate(&e));
-        const void* A = p->prec == gs::kFP64 ? p->at64.p : p->amat.p;
-        p->ls.dz = dz;
         for (int k = 0; k < n; ++k) {
             hipEvent_t* e = &ev[(size_t)5 * k];
             HCK(hipEventRecord(e[0], p->st));

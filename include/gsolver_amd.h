@@ -141,7 +141,8 @@ int gs_fft_lines(gs_plan* p, const double* in, double* out, int n_lines, int sig
 /* n steps of gs_fixed_step enqueued back to back on the plan's stream with one
  * synchronisation at the end (the fixed-dz throughput mode of the benchmark;
  * the reference has no counterpart).  ms4 (may be NULL) receives the mean
- * device time per step of the four kernel groups, as gs_last_timings. */
+ * device time per step of the four kernel groups, as gs_last_timings; with
+ * ms4 NULL no timing events are recorded between the kernels. */
 int gs_fixed_steps(gs_plan* p, double dz, int n, double* ms4);
 
 /* Reverse mode (no counterpart in the reference, which has no adjoint).

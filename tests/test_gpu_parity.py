@@ -174,17 +174,19 @@ def test_fixed_steps_production_line_length(tmp_path, prec):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
 def test_fixed_steps_batch_equals_single(tmp_path, prec):
-    """gs_fixed_steps (one synchronisation) runs the same kernels as repeated
-    gs_fixed_step: identical fields."""
+    """gs_fixed_steps (one synchronisation), with or without timing events, runs
+    the same kernels as repeated gs_fixed_step: identical fields."""
     cfg = write_cfg(tmp_path / "c.txt", **HOT)
-    p1, p2 = gsa.Plan(cfg, precision=prec), gsa.Plan(cfg, precision=prec)
+    p1, p2, p3 = (gsa.Plan(cfg, precision=prec) for _ in range(3))
     for _ in range(3):
         p1.fixed_step(2e-6)
     t = p2.fixed_steps(2e-6, 3)
     assert set(t) == {"nl_rk4", "qdht_lstep", "qdht_back", "reduce"} and all(v > 0 for v in t.values())
-    np.testing.assert_array_equal(p1.get(), p2.get())
-    np.testing.assert_array_equal(p1.get(gsa.B2), p2.get(gsa.B2))
-    assert p2.scalar("STEP_N") == p1.scalar("STEP_N") == 3
+    assert p3.fixed_steps(2e-6, 3, timings=False) is None
+    for p in (p2, p3):
+        np.testing.assert_array_equal(p1.get(), p.get())
+        np.testing.assert_array_equal(p1.get(gsa.B2), p.get(gsa.B2))
+        assert p.scalar("STEP_N") == p1.scalar("STEP_N") == 3
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
