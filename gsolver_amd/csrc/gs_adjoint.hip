@@ -4,8 +4,8 @@
 // forward scheme of gs_line.hip / gs_qdht.hip (calculate_NLresponse,
 // propagator.cu:629-783; the plasma Heun recurrence, ionization.cpp:9-95; the
 // RK4 of propagator.cu:376-425; calculate_Lstep, propagator.cu:570-625) with
-// the step sizes held fixed.  The QDHT transpose diag(1/m1) C diag(m1) runs on
-// the forward GEMM kernels with its own matrix.
+// the step sizes held fixed.  The QDHT transpose diag(1/m1) C^T diag(m1) runs
+// on the forward GEMM kernels with its own operator.
 //
 // Cotangent of a complex value z of a real objective L: dL/dRe z + i dL/dIm z;
 // a complex-linear map y = M x has g_x = M^H g_y.  Per RHS evaluation

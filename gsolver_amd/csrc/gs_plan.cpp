@@ -194,6 +194,27 @@ void download_field64(gs_plan* p, const void* src, double* host, size_t nlines)
     sync(p);
 }
 
+// split-BF16 operator M (row-major [m][k], rp x rp) of the mixed GEMM: hi = bf16(a),
+// lo = bf16(a - hi), in the MFMA fragment order of the GEMM: [m/128][k/32][k slice
+// (k%32)/16][32-row block][hi, lo][lane][8], lane = m%32 + 32 ((k%16)/8), element
+// k%8 (v_mfma_f32_32x32x16_bf16 A operand)
+void upload_bf16_fragments(DevBuf& dst, const std::vector<double>& M, int rp)
+{
+    std::vector<uint16_t> fr((size_t)2 * rp * rp);
+    const int nk = rp / 32;
+    for (int m = 0; m < rp; ++m)
+        for (int k = 0; k < rp; ++k) {
+            const float a = (float)M[(size_t)m * rp + k];
+            const uint16_t h = bf16_rn_host(a), l = bf16_rn_host(a - bf16_to_float(h));
+            const int mt = m / 128, rbb = (m % 128) / 32, step = k / 32, kk = (k % 32) / 16;
+            const int lane = m % 32 + 32 * ((k % 16) / 8), e = k % 8;
+            const size_t base = (((((size_t)mt * nk + step) * 2 + kk) * 4 + rbb) * 2) * 64 * 8;
+            fr[base + lane * 8 + e] = h;
+            fr[base + 64 * 8 + lane * 8 + e] = l;
+        }
+    upload(dst, fr.data(), fr.size() * 2);
+}
+
 void build_plan(gs_plan* p)
 {
     const gs::Setup& s = p->s;
@@ -224,22 +245,7 @@ void build_plan(gs_plan* p)
         std::vector<float> f(At.begin(), At.end());
         upload(p->amat, f.data(), f.size() * 4);
     } else {
-        // split-BF16: hi = bf16(a), lo = bf16(a - hi), in the MFMA fragment order of
-        // the GEMM: [m/128][k/32][k slice (k%32)/16][32-row block][hi, lo][lane][8],
-        // lane = m%32 + 32 ((k%16)/8), element k%8 (v_mfma_f32_32x32x16_bf16 A operand)
-        std::vector<uint16_t> fr((size_t)2 * rp * rp);
-        const int nk = rp / 32;
-        for (int m = 0; m < rp; ++m)
-            for (int k = 0; k < rp; ++k) {
-                const float a = (float)A[(size_t)m * rp + k];
-                const uint16_t h = bf16_rn_host(a), l = bf16_rn_host(a - bf16_to_float(h));
-                const int mt = m / 128, rbb = (m % 128) / 32, step = k / 32, kk = (k % 32) / 16;
-                const int lane = m % 32 + 32 * ((k % 16) / 8), e = k % 8;
-                const size_t base = (((((size_t)mt * nk + step) * 2 + kk) * 4 + rbb) * 2) * 64 * 8;
-                fr[base + lane * 8 + e] = h;
-                fr[base + 64 * 8 + lane * 8 + e] = l;
-            }
-        upload(p->amat, fr.data(), fr.size() * 2);
+        upload_bf16_fragments(p->amat, A, rp);
     }
 
     // ---- tables
@@ -405,11 +411,11 @@ void build_plan(gs_plan* p)
 }
 
 // ---- autotuner of the line kernel's samples per thread
-// For FP32 / mixed lines of N_T >= 1024 the RK4 kernel has three shapes (4, 8
-// or 16 samples per thread: N_T/4, N_T/8 or N_T/16 threads per line, different
+// For FP32 / mixed lines of N_T >= 1024 the RK4 kernel has two shapes (8 or 16
+// samples per thread: 512 or 256 threads per line at N_T = 4096, different
 // register and LDS footprints).  The first plan of a (precision, N_T, N_R) in
-// the process times all of them on its initial field and keeps the fastest
-// when it beats 8 by 5 %; GS_NL_E overrides.
+// the process times both on its initial field and keeps 16 when it beats 8 by
+// 5 %; GS_NL_E overrides (4 is also accepted).
 void autotune_nl(gs_plan* p)
 {
     static std::mutex mu;
@@ -430,10 +436,10 @@ void autotune_nl(gs_plan* p)
     HCK(hipEventCreate(&e1));
     // candidates alternate (warm-up round first), each keeps its best time; the
     // default shape (8) is replaced only by a clear (5 %) win
-    const int cand[3] = {8, 16, 4};
-    float tmin[3] = {1e30f, 1e30f, 1e30f};
+    const int cand[2] = {8, 16};
+    float tmin[2] = {1e30f, 1e30f};
     for (int rep = 0; rep < 6; ++rep)
-        for (int c = 0; c < 3; ++c) {
+        for (int c = 0; c < 2; ++c) {
             a.nl_e = cand[c];
             HCK(hipEventRecord(e0, p->st));
             LCK(gs::launch_nl_rk4(p->prec, p->s.NT, p->field.p, p->b1.p, p->ZSTEP, a, p->linemax.as<double>(), p->st));
@@ -445,7 +451,7 @@ void autotune_nl(gs_plan* p)
         }
     int best = cand[0];
     float tb = 0.95f * tmin[0];
-    for (int c = 1; c < 3; ++c)
+    for (int c = 1; c < 2; ++c)
         if (tmin[c] < tb) {
             tb = tmin[c];
             best = cand[c];
@@ -1243,16 +1249,22 @@ int gs_adjoint_forward(gs_plan* p, int n, const double* dz_rk, const double* dz_
         }
         p->adj_seg = K;
         if (!p->adjmat.p) {
-            // row-major A = diag(m1) C diag(1/m1), padded: the GEMM reads its operand
-            // transposed, so this matrix makes it apply A^T = diag(1/m1) C^T diag(m1)
+            // the transposed operator A^T = diag(1/m1) C^T diag(m1) for the plan's GEMM:
+            // FP64 / FP32 take their operand transposed, so they get row-major A;
+            // mixed gets the split-BF16 fragments of A^T
             std::vector<double> A((size_t)rp * rp, 0.0);
             for (int i = 0; i < nr; ++i)
                 for (int j = 0; j < nr; ++j) A[(size_t)i * rp + j] = p->s.qm1[i] * p->s.qC[(size_t)i * nr + j] / p->s.qm1[j];
             if (p->prec == gs::kFP64) {
                 upload(p->adjmat, A.data(), A.size() * 8);
-            } else {
+            } else if (p->prec == gs::kFP32) {
                 std::vector<float> f(A.begin(), A.end());
                 upload(p->adjmat, f.data(), f.size() * 4);
+            } else {
+                std::vector<double> At((size_t)rp * rp);
+                for (int i = 0; i < rp; ++i)
+                    for (int j = 0; j < rp; ++j) At[(size_t)i * rp + j] = A[(size_t)j * rp + i];
+                upload_bf16_fragments(p->adjmat, At, rp);
             }
             for (DevBuf* b : {&p->gA, &p->gB, &p->gT1, &p->gT2}) {
                 b->alloc(fb);
@@ -1293,7 +1305,7 @@ int gs_adjoint_backward(gs_plan* p, const double* gfield, const double* gb2, int
         if (n == 0) return fail(GS_E_ARG, "gs_adjoint_backward: no steps recorded (call gs_adjoint_forward first)");
         const size_t fb = p->field_bytes();
         const int rp = p->rp, nt = p->s.NT, K = p->adj_seg;
-        const gs::Prec pa = p->prec == gs::kFP64 ? gs::kFP64 : gs::kFP32;   // precision of the transposed GEMMs
+        const gs::Prec pa = p->prec;     // the transposed GEMMs run in the plan's GEMM precision
         const void* A = p->prec == gs::kFP64 ? p->at64.p : p->amat.p;
         upload_field64(p, p->gA, gfield);
         if (gb2) upload_field64(p, p->gB, gb2);
