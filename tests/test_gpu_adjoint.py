@@ -74,6 +74,31 @@ def test_adjoint_segmented_matches_single_segment(tmp_path, prec):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("case", ["no_raman", "plasma_only", "kerr_only"])
+def test_adjoint_physics_variants_vs_oracle(tmp_path, case):
+    """The transposed march with parts of the RHS switched off (FP64, 3 steps):
+    no Raman response (RAMAN_FRACTION 0: y = I), Kerr negligible (the plasma
+    and loss transposes alone), and a field below the ionization threshold
+    (Kerr and Raman alone, plasma terms skipped)."""
+    kw = dict(HOT)
+    if case == "no_raman":
+        kw["RAMAN_FRACTION"] = "0"
+    elif case == "plasma_only":
+        kw["NONLIN_REFRINDEX"] = "1e-30"
+    else:
+        kw["ENERGY"] = "1e-7"
+    cfg = write_cfg(tmp_path / "c.txt", **kw)
+    p, o = gsa.Plan(cfg, precision="fp64"), oracle.Sim(cfg, nthreads=8)
+    rng = np.random.default_rng(12)
+    gF, gB = crand(rng, (p.R, p.T)), crand(rng, (p.R, p.T))
+    g0_ref, b0_ref = o.adjoint_march(DZ_RK, DZ_LIN, gF, gB)
+    p.adjoint_forward(DZ_RK, DZ_LIN)
+    g0, b0 = p.adjoint_backward(gF, gB, tie_b2=False)
+    e1, e2 = rel_l2(g0, g0_ref), rel_l2(b0, b0_ref)
+    print("adjoint", case, e1, e2)
+    assert e1 < 1e-10 and e2 < 1e-10
+
+
 def test_adjoint_tied_b2(tmp_path):
     """tie_b2 folds the initial BIGBUFFER2 = QDHT(FIELD) cotangent into FIELD's:
     gfield0(tied) = gfield0 + Q^T gb2_0, Q = diag(m1) C diag(1/m1)."""
