@@ -21,8 +21,18 @@
 #include <tgmath.h>
 #include "o_internal.h"
 
+/* A throw inside a worker of o_parallel returns to that worker's own frame
+ * (o_tl_jb, with the message in o_tl_err); o_parallel re-throws on the
+ * calling thread after every worker has been joined. */
+static __thread jmp_buf* o_tl_jb = NULL;
+static __thread char* o_tl_err = NULL;
+
 void o_throw(orc_sim* s, const char* msg)
 {
+    if (o_tl_jb) {
+        snprintf(o_tl_err, sizeof s->err, "%s", msg);
+        longjmp(*o_tl_jb, 1);
+    }
     if (msg != s->err) snprintf(s->err, sizeof s->err, "%s", msg);
     if (s->jb_armed) longjmp(s->jb, 1);
     fprintf(stderr, "oracle error outside API frame: %s\n", msg);
@@ -47,6 +57,8 @@ typedef struct {
     int tid, nth, n;
     void (*fn)(orc_sim*, int tid, int i0, int i1, void* ctx);
     void* ctx;
+    int failed;
+    char err[512];
 } o_job;
 
 static void* o_job_run(void* a)
@@ -55,7 +67,18 @@ static void* o_job_run(void* a)
     int chunk = (j->n + j->nth - 1) / j->nth;
     int i0 = j->tid * chunk, i1 = i0 + chunk;
     if (i1 > j->n) i1 = j->n;
-    if (i0 < i1) j->fn(j->s, j->tid, i0, i1, j->ctx);
+    jmp_buf jb;
+    jmp_buf* prev_jb = o_tl_jb;
+    char* prev_err = o_tl_err;
+    o_tl_jb = &jb;
+    o_tl_err = j->err;
+    if (setjmp(jb) == 0) {
+        if (i0 < i1) j->fn(j->s, j->tid, i0, i1, j->ctx);
+    } else {
+        j->failed = 1;
+    }
+    o_tl_jb = prev_jb;
+    o_tl_err = prev_err;
     return NULL;
 }
 
@@ -64,14 +87,16 @@ void o_parallel(orc_sim* s, int n, void (*fn)(orc_sim*, int, int, int, void*), v
     int nth = s->nthreads;
     if (nth <= 1 || n < 2) { fn(s, 0, 0, n, ctx); return; }
     pthread_t th[256];
-    o_job jobs[256];
+    static __thread o_job jobs[256];
     if (nth > 256) nth = 256;
     for (int t = 0; t < nth; ++t) {
-        jobs[t] = (o_job){s, t, nth, n, fn, ctx};
+        jobs[t] = (o_job){s, t, nth, n, fn, ctx, 0, ""};
         if (t) pthread_create(&th[t], NULL, o_job_run, &jobs[t]);
     }
     o_job_run(&jobs[0]);
     for (int t = 1; t < nth; ++t) pthread_join(th[t], NULL);
+    for (int t = 0; t < nth; ++t)
+        if (jobs[t].failed) o_throw(s, jobs[t].err);
 }
 
 /* ---- ionization ---- */

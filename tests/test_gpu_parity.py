@@ -321,3 +321,38 @@ def test_cli_drop_in_matches_library_march(tmp_path):
     bad = subprocess.run([exe, str(tmp_path / "missing.txt")], cwd=str(tmp_path), capture_output=True, text=True,
                          timeout=120)
     assert bad.returncode == 2 and "Error" in bad.stdout
+
+
+@pytest.mark.parametrize("driver", ["host", "device"])
+def test_blast_error_matches_reference_message(tmp_path, monkeypatch, driver):
+    """An intensity beyond the ionization table (ln I > 57, here a 1 kJ pulse)
+    stops the march with the reference's throw (ionization.cpp:114-118),
+    GS_E_PHYSICS, from the host-driven and the device-driven controller; the
+    kernels only set a status bit and exit normally."""
+    if driver == "host":
+        monkeypatch.setenv("GS_HOST_MARCH", "1")
+    cfg = write_cfg(tmp_path / "c.txt", **dict(HOT, ENERGY="1e3", N_T="256"))
+    p = gsa.Plan(cfg, precision="mixed")
+    with pytest.raises(gsa.GsError, match="Keep away, blast is possible") as ei:
+        p.march(str(tmp_path))
+    assert ei.value.code == -4
+    o = oracle.Sim(cfg, nthreads=4)
+    with pytest.raises(oracle.OracleError, match="Keep away, blast is possible"):
+        o.propagator()
+
+
+@pytest.mark.parametrize("driver", ["host", "device"])
+def test_step_collapse_error_matches_reference_message(tmp_path, monkeypatch, driver):
+    """A step-size request below 1e-10 of the plane spacing (MAX_TOLERANCE 1e-13)
+    stops the march with "Minimum Z step size reached, collapse is possible!"
+    (propagator.cu:159-163) in both controllers, as the oracle does."""
+    if driver == "host":
+        monkeypatch.setenv("GS_HOST_MARCH", "1")
+    cfg = write_cfg(tmp_path / "c.txt", **dict(HOT, MAX_TOLERANCE="1e-13", N_T="256"))
+    p = gsa.Plan(cfg, precision="mixed")
+    with pytest.raises(gsa.GsError, match="Minimum Z step size reached, collapse is possible!") as ei:
+        p.march(str(tmp_path))
+    assert ei.value.code == -4
+    o = oracle.Sim(cfg, nthreads=4)
+    with pytest.raises(oracle.OracleError, match="Minimum Z step size reached"):
+        o.propagator()

@@ -140,3 +140,17 @@ def test_rk4_combine(small):
     assert np.abs(B1[r] - ref).max() <= 1e-13 * np.abs(ref).max()
     assert maxA2 == np.max(np.abs(F) ** 2)
     assert abs(maxNL2 - np.max(np.abs(B1) ** 2)) <= 1e-15 * maxNL2
+
+
+@pytest.mark.parametrize("kw,msg", [(dict(ENERGY="1e3"), "blast is possible"),
+                                    (dict(MAX_TOLERANCE="1e-13"), "collapse is possible")])
+def test_runtime_throws_from_worker_threads(tmp_path, kw, msg):
+    """The reference's run-time throws raised inside the line-parallel NL step
+    (four threads) reach the API call as errors: the blast of ionization.cpp:
+    114-118 for a 1 kJ pulse, the step collapse of propagator.cu:159-163 for
+    MAX_TOLERANCE 1e-13; the simulation stays usable for a new call."""
+    cfg = make_cfg(str(tmp_path / "c.txt"), **dict(dict(ENERGY="20e-6", DIAMETER_FWHM="4e-5", R_MAX="0.2e-3"), **kw))
+    s = oracle.Sim(cfg, nthreads=4)
+    for _ in range(2):
+        with pytest.raises(oracle.OracleError, match=msg):
+            s.propagator()
