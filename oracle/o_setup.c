@@ -50,7 +50,7 @@ int o_cfg_find(orc_sim* s, const char* name, const char** val)
 
 static void missing(orc_sim* s, const char* name)
 {
-    snprintf(s->err, sizeof s->err, "load_namedparameter: parameter %s not found", name);
+    snprintf(s->err, sizeof s->err, "load_namedparameter: parameter %.400s not found", name);
     o_throw(s, s->err);
 }
 
