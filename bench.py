@@ -271,7 +271,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=4,
+                    help="fixed-dz steps of the FP64 oracle timed for cpu_baseline (about 3 s each on 16 threads)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the adaptive-march accuracy check")
     ap.add_argument("--batch-runs", type=int, default=16, help="runs of the energy/width scan (0: skip)")
     ap.add_argument("--batch-zmax", type=float, default=5e-6, help="distance of each scan run in m")
