@@ -348,7 +348,7 @@ __device__ __forceinline__ void lstep_phasor(typename Pol::P ph, typename Pol::R
 }
 
 // one point of calculate_Lstep: B2 += dz B1k g; FIELD = B2 e^{i dz rate} amp; B2 = FIELD,
-// with amp = e^{dz Im k(omega)} of the point's column
+// with amp = e^{dz Im k(omega)} of the point's column; B2 lives in fk (LstepArgs)
 template <class Pol>
 __device__ __forceinline__ void lstep_point(size_t o, typename Pol::R yr, typename Pol::R yi, typename Pol::R amp,
                                             const LstepArgs& la)
@@ -356,22 +356,19 @@ __device__ __forceinline__ void lstep_point(size_t o, typename Pol::R yr, typena
     using R = typename Pol::R;
     using P = typename Pol::P;
     C2<R>* fk = (C2<R>*)la.fk;
-    C2<R>* b2 = (C2<R>*)la.b2;
     const R g = ((const R*)la.gfac)[o];
     if (g < (R)0) { fk[o] = C2<R>{0, 0}; return; }
     R sn, cs;
     lstep_phasor<Pol>((P)la.dz * ((const P*)la.rate)[o], &sn, &cs);
-    const C2<R> b = b2[o];
+    const C2<R> b = fk[o];
     const R dg = (R)la.dz * g;
     const R br = b.x + dg * yr, bi = b.y + dg * yi;
-    const C2<R> f = {amp * (br * cs - bi * sn), amp * (br * sn + bi * cs)};
-    fk[o] = f;
-    b2[o] = f;
+    fk[o] = C2<R>{amp * (br * cs - bi * sn), amp * (br * sn + bi * cs)};
 }
 
-// lstep_point over NB points with all table loads issued before any store
-// (the stores could alias later loads, so a per-point loop would serialise
-// one memory round trip per point)
+// lstep_point over NB points with all table and state loads issued before any
+// store (the stores could alias later loads, so a per-point loop would
+// serialise one memory round trip per point)
 template <class Pol, int NB>
 __device__ __forceinline__ void lstep_batch(const size_t (&o)[NB], const typename Pol::R (&yr)[NB],
                                             const typename Pol::R (&yi)[NB], typename Pol::R amp, const LstepArgs& la)
@@ -379,7 +376,6 @@ __device__ __forceinline__ void lstep_batch(const size_t (&o)[NB], const typenam
     using R = typename Pol::R;
     using P = typename Pol::P;
     C2<R>* fk = (C2<R>*)la.fk;
-    C2<R>* b2 = (C2<R>*)la.b2;
     R g[NB];
     P rt[NB];
     C2<R> b[NB];
@@ -387,7 +383,7 @@ __device__ __forceinline__ void lstep_batch(const size_t (&o)[NB], const typenam
     for (int j = 0; j < NB; ++j) {
         g[j] = ((const R*)la.gfac)[o[j]];
         rt[j] = ((const P*)la.rate)[o[j]];
-        b[j] = b2[o[j]];
+        b[j] = fk[o[j]];
     }
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
@@ -399,9 +395,7 @@ __device__ __forceinline__ void lstep_batch(const size_t (&o)[NB], const typenam
         lstep_phasor<Pol>((P)la.dz * rt[j], &sn, &cs);
         const R dg = (R)la.dz * g[j];
         const R br = b[j].x + dg * yr[j], bi = b[j].y + dg * yi[j];
-        const C2<R> f = {amp * (br * cs - bi * sn), amp * (br * sn + bi * cs)};
-        fk[o[j]] = f;
-        b2[o[j]] = f;
+        fk[o[j]] = C2<R>{amp * (br * cs - bi * sn), amp * (br * sn + bi * cs)};
     }
 }
 

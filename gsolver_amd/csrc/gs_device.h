@@ -72,13 +72,16 @@ struct NLArgs {
 //   rate[i][w] = -HO[w] - s k1[w],  g[i][w] = 1/(1+s),  s = sqrtHO(-kt2_i / k1_w^2)
 // with g < 0 marking points where FIELD is zeroed and B2 left alone (omega
 // outside [OMEGA_MIN, OMEGA_MAX], kt2 > MAX_KT2 Re(k^2), padding rows).
+// After every linear step B2 equals FIELD_k on the other points, and B2 never
+// changes on the masked ones, so the device keeps one array: fk holds FIELD_k
+// (zero where masked) and is the B2 the next step updates; the plan's b2
+// buffer keeps only the masked points' values, merged back for gs_get.
 struct LstepArgs {
     double dz;
     const double* k2;           // Im k(omega)
     const void* rate;           // [rp][T], P precision of the policy
     const void* gfac;           // [rp][T], field precision
-    void* b2;                   // BIGBUFFER2 (C2<R>), updated in place
-    void* fk;                   // FIELD in k-space (C2<R>), written
+    void* fk;                   // FIELD in k-space = B2 on unmasked points (C2<R>), updated in place
     const DevCtl* ctl;          // device-driven march: dz = ctl->dz_lin; skip unless accepted
 };
 
@@ -112,6 +115,9 @@ int launch_line_diag(Prec p, int nt, const void* field, int nlines, const double
 // MFMA fragment order (see the plan set-up).
 int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* in, void* out,
                 const LstepArgs* lstep, hipStream_t st, const DevCtl* gate = nullptr);
+// BIGBUFFER2 <-> FIELD_k: out = masked ? b2 : fk (the full B2), or fk = masked ? 0 : b2
+int launch_b2_merge(Prec p, const void* fk, const void* b2, void* out, const void* gfac, size_t n_complex, hipStream_t st);
+int launch_b2_split(Prec p, const void* b2, void* fk, const void* gfac, size_t n_complex, hipStream_t st);
 // element-wise linear step from a k-space increment (calculate_Lstep)
 int launch_lstep(Prec p, int rp, int nt, const void* b1k, const LstepArgs& la, hipStream_t st);
 // max of the per-line maxima -> status

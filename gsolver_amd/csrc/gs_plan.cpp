@@ -99,7 +99,8 @@ struct gs_plan {
     hipStream_t st = nullptr;
     hipEvent_t ev[5] = {};
 
-    DevBuf field, b1, b2, fk, tmp;       // [rp][T] in field precision
+    DevBuf field, b1, b2, fk, tmp;       // [rp][T] in field precision; fk = FIELD_k = B2 on the
+                                         // unmasked points (the live state), b2 = B2's masked points
     DevBuf amat, at64;                   // QDHT operators
     DevBuf tw, tw64, zpow, pf, kerr, ionrate, k2, lrate, lgfac, avgw;
     DevBuf linemax, status, diag, et, rhot, scratch64a, scratch64b;
@@ -119,12 +120,13 @@ struct gs_plan {
     hipGraphExec_t slot_exec = nullptr;
 
     // reverse mode: FIELD before each step of the segment being reversed
-    // (ckpt), FIELD and BIGBUFFER2 at the start of every segment (snaps), the
+    // (ckpt), FIELD and FIELD_k (the live BIGBUFFER2) at the start of every
+    // segment (snaps), the
     // replay buffers that recompute a segment, the step sizes, the QDHT
     // transpose (row-major A, so the transposed-operand GEMM computes A^T x)
     // and the cotangent buffers.  A march whose per-step checkpoints fit in
     // adj_budget bytes is one segment (no recomputation).
-    DevBuf ckpt, snaps, rfield, rb1, rb2, rfk, adjmat, gA, gB, gT1, gT2, stage;
+    DevBuf ckpt, snaps, rfield, rb1, rfk, adjmat, gA, gB, gT1, gT2, stage;
     std::vector<double> adj_rk, adj_lin;
     int adj_seg = 0;                       // steps per segment
     double adj_budget = 16.0 * (1 << 30);  // bytes of per-step checkpoints kept without recomputation
@@ -371,7 +373,6 @@ void build_plan(gs_plan* p)
     l.k2 = p->k2.as<double>();
     l.rate = p->lrate.p;
     l.gfac = p->lgfac.p;
-    l.b2 = p->b2.p;
     l.fk = p->fk.p;
 
     // ---- initial field: FFT of E(r,t), spectral phase, /FIELD_DENOM (pulse_gg.cpp:100-121, process_input.cpp:58)
@@ -404,6 +405,7 @@ void build_plan(gs_plan* p)
         HCK(hipMemcpyAsync(p->scratch64a.p, sp.data(), n * 16, hipMemcpyHostToDevice, p->st));
         LCK(gs::launch_qdht(gs::kFP64, rp, nt, p->at64.p, p->scratch64a.p, p->scratch64b.p, nullptr, p->st));
         LCK(gs::launch_convert(p->prec, p->scratch64b.as<double>(), p->b2.p, (size_t)rp * nt, 1, p->st));
+        LCK(gs::launch_b2_split(p->prec, p->b2.p, p->fk.p, p->lgfac.p, (size_t)rp * nt, p->st));
         sync(p);
     }
     p->s.field_t.clear();
@@ -893,7 +895,13 @@ int gs_get(gs_plan* p, int buffer, double* host)
 {
     return guarded([&]() {
         if (!p || !host) return fail(GS_E_ARG, "null argument");
-        download_field64(p, p->buf(buffer), host, p->s.NR);
+        void* src = p->buf(buffer);
+        if (buffer == GS_B2) {
+            // B2 = FIELD_k on the unmasked points, its set-up values on the masked ones
+            LCK(gs::launch_b2_merge(p->prec, p->fk.p, p->b2.p, p->tmp.p, p->lgfac.p, (size_t)p->rp * p->s.NT, p->st));
+            src = p->tmp.p;
+        }
+        download_field64(p, src, host, p->s.NR);
         return 0;
     });
 }
@@ -908,6 +916,8 @@ int gs_put(gs_plan* p, int buffer, const double* host)
         upload(tmp, host, n * 16);
         HCK(hipMemsetAsync(dst, 0, p->field_bytes(), p->st));
         LCK(gs::launch_convert(p->prec, tmp.as<double>(), dst, n, 1, p->st));
+        if (buffer == GS_B2)   // the unmasked part is the state the next linear step updates
+            LCK(gs::launch_b2_split(p->prec, p->b2.p, p->fk.p, p->lgfac.p, (size_t)p->rp * p->s.NT, p->st));
         sync(p);
         return 0;
     });
@@ -949,9 +959,17 @@ int gs_qdht(gs_plan* p, int buffer)
     return guarded([&]() {
         if (!p) return fail(GS_E_ARG, "null plan");
         void* b = p->buf(buffer);
-        LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, p->prec == gs::kFP64 ? p->at64.p : p->amat.p, b,
-                            p->tmp.p, nullptr, p->st));
-        HCK(hipMemcpyAsync(b, p->tmp.p, p->field_bytes(), hipMemcpyDeviceToDevice, p->st));
+        const size_t n = (size_t)p->rp * p->s.NT;
+        const void* A = p->prec == gs::kFP64 ? p->at64.p : p->amat.p;
+        if (buffer == GS_B2) {
+            // the full B2 in, the full B2 out, then its unmasked part back into FIELD_k
+            LCK(gs::launch_b2_merge(p->prec, p->fk.p, p->b2.p, p->tmp.p, p->lgfac.p, n, p->st));
+            LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->tmp.p, p->b2.p, nullptr, p->st));
+            LCK(gs::launch_b2_split(p->prec, p->b2.p, p->fk.p, p->lgfac.p, n, p->st));
+        } else {
+            LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, b, p->tmp.p, nullptr, p->st));
+            HCK(hipMemcpyAsync(b, p->tmp.p, p->field_bytes(), hipMemcpyDeviceToDevice, p->st));
+        }
         sync(p);
         return 0;
     });
@@ -1239,7 +1257,7 @@ int gs_adjoint_forward(gs_plan* p, int n, const double* dz_rk, const double* dz_
         if (p->ckpt.n != fb * (size_t)K) p->ckpt.alloc(fb * (size_t)K);
         if (K < n) {
             if (p->snaps.n != 2 * fb * (size_t)nseg) p->snaps.alloc(2 * fb * (size_t)nseg);
-            for (DevBuf* b : {&p->rfield, &p->rb1, &p->rb2, &p->rfk})
+            for (DevBuf* b : {&p->rfield, &p->rb1, &p->rfk})
                 if (b->n != fb) {
                     b->alloc(fb);
                     HCK(hipMemset(b->p, 0, fb));
@@ -1280,7 +1298,7 @@ int gs_adjoint_forward(gs_plan* p, int n, const double* dz_rk, const double* dz_
             } else if (k % K == 0) {
                 char* sn = (char*)p->snaps.p + (size_t)(k / K) * 2 * fb;
                 HCK(hipMemcpyAsync(sn, p->field.p, fb, hipMemcpyDeviceToDevice, p->st));
-                HCK(hipMemcpyAsync(sn + fb, p->b2.p, fb, hipMemcpyDeviceToDevice, p->st));
+                HCK(hipMemcpyAsync(sn + fb, p->fk.p, fb, hipMemcpyDeviceToDevice, p->st));
             }
             LCK(gs::launch_nl_rk4(p->prec, nt, p->field.p, p->b1.p, dz_rk[k], p->nl, p->linemax.as<double>(), p->st));
             ls.dz = dz_lin[k];
@@ -1311,8 +1329,7 @@ int gs_adjoint_backward(gs_plan* p, const double* gfield, const double* gb2, int
         if (gb2) upload_field64(p, p->gB, gb2);
         else HCK(hipMemsetAsync(p->gB.p, 0, fb, p->st));
         gs::LstepArgs la = p->ls;
-        gs::LstepArgs lr = p->ls;        // the replay's linear step writes the replay buffers
-        lr.b2 = p->rb2.p;
+        gs::LstepArgs lr = p->ls;        // the replay's linear step updates the replay buffers
         lr.fk = p->rfk.p;
         for (int k1 = n; k1 > 0;) {
             const int k0 = (k1 - 1) / K * K;
@@ -1320,7 +1337,7 @@ int gs_adjoint_backward(gs_plan* p, const double* gfield, const double* gb2, int
                 // recompute the segment's FIELDs from its snapshot (same kernels, same order)
                 const char* sn = (const char*)p->snaps.p + (size_t)(k0 / K) * 2 * fb;
                 HCK(hipMemcpyAsync(p->rfield.p, sn, fb, hipMemcpyDeviceToDevice, p->st));
-                HCK(hipMemcpyAsync(p->rb2.p, sn + fb, fb, hipMemcpyDeviceToDevice, p->st));
+                HCK(hipMemcpyAsync(p->rfk.p, sn + fb, fb, hipMemcpyDeviceToDevice, p->st));
                 for (int k = k0; k < k1; ++k) {
                     HCK(hipMemcpyAsync((char*)p->ckpt.p + (size_t)(k - k0) * fb, p->rfield.p, fb, hipMemcpyDeviceToDevice, p->st));
                     if (k + 1 == k1) break;    // the segment's last FIELD is all the reverse sweep needs

@@ -507,6 +507,22 @@ __global__ void k_convert_back(const C2<R>* __restrict__ src, double* __restrict
     }
 }
 
+// the full BIGBUFFER2 from FIELD_k (unmasked points) and the masked points' values
+template <class R>
+__global__ void k_b2_merge(const C2<R>* __restrict__ fk, const C2<R>* __restrict__ b2, C2<R>* __restrict__ out,
+                           const R* __restrict__ g, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = g[i] < (R)0 ? b2[i] : fk[i];
+}
+// FIELD_k (the live B2) from a full BIGBUFFER2: zero on the masked points
+template <class R>
+__global__ void k_b2_split(const C2<R>* __restrict__ b2, C2<R>* __restrict__ fk, const R* __restrict__ g, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) fk[i] = g[i] < (R)0 ? C2<R>{0, 0} : b2[i];
+}
+
 template <class R>
 __global__ void k_avg_spectrum(int nr, int nt, const C2<R>* __restrict__ f, const double* __restrict__ wts, double* out)
 {
@@ -591,6 +607,28 @@ int launch_convert(Prec p, const double* src64, void* dst, size_t n, int to_devi
         if (to_device) hipLaunchKernelGGL(k_convert<float>, grid, dim3(256), 0, st, src64, (C2<float>*)dst, n);
         else hipLaunchKernelGGL(k_convert_back<float>, grid, dim3(256), 0, st, (const C2<float>*)dst, (double*)src64, n);
     }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int launch_b2_merge(Prec p, const void* fk, const void* b2, void* out, const void* gfac, size_t n, hipStream_t st)
+{
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (p == kFP64)
+        hipLaunchKernelGGL(k_b2_merge<double>, grid, dim3(256), 0, st, (const C2<double>*)fk, (const C2<double>*)b2, (C2<double>*)out, (const double*)gfac, n);
+    else
+        hipLaunchKernelGGL(k_b2_merge<float>, grid, dim3(256), 0, st, (const C2<float>*)fk, (const C2<float>*)b2, (C2<float>*)out, (const float*)gfac, n);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int launch_b2_split(Prec p, const void* b2, void* fk, const void* gfac, size_t n, hipStream_t st)
+{
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (p == kFP64)
+        hipLaunchKernelGGL(k_b2_split<double>, grid, dim3(256), 0, st, (const C2<double>*)b2, (C2<double>*)fk, (const double*)gfac, n);
+    else
+        hipLaunchKernelGGL(k_b2_split<float>, grid, dim3(256), 0, st, (const C2<float>*)b2, (C2<float>*)fk, (const float*)gfac, n);
     HIPCHK(hipGetLastError());
     return 0;
 }

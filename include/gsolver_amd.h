@@ -54,7 +54,9 @@ enum gs_buffer {
     GS_FIELD = 0,    /* FIELD:      the field in (r, omega) */
     GS_B1 = 1,       /* BIGBUFFER1: NL increment (r, omega); k-space after the first QDHT */
     GS_B2 = 2,       /* BIGBUFFER2: the field carried across steps in (k_perp, omega) */
-    GS_FK = 3        /* FIELD in (k_perp, omega) after the linear step, before the QDHT back */
+    GS_FK = 3        /* FIELD in (k_perp, omega) after the linear step, before the QDHT back;
+                        equal to BIGBUFFER2 except where the linear step zeroes FIELD (there
+                        BIGBUFFER2 keeps its value), so putting it also sets BIGBUFFER2 there */
 };
 
 /* Set-up from a reference config file: replaces process_input()
