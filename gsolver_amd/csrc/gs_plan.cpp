@@ -129,7 +129,8 @@ struct gs_plan {
     DevBuf ckpt, snaps, rfield, rb1, rfk, adjmat, gA, gB, gT1, gT2, stage;
     std::vector<double> adj_rk, adj_lin;
     int adj_seg = 0;                       // steps per segment
-    double adj_budget = 16.0 * (1 << 30);  // bytes of per-step checkpoints kept without recomputation
+    double adj_budget = -1;                // bytes of per-step checkpoints kept without recomputation
+                                           // (< 0: 60 % of the device memory free at the forward)
     double last_dz_nl = 0;     // NL step size of the last accepted gs_step attempt
 
     void* buf(int id)
@@ -837,8 +838,8 @@ int gs_set_scalar(gs_plan* p, const char* name, double v)
     else if (n == "n_Z") p->n_Z = (int)v;
     else if (n == "MAX_TOLERANCE") p->s.MAX_TOLERANCE = v;
     else if (n == "ADJOINT_CHECKPOINT_BYTES") {
-        if (!(v >= 0)) return fail(GS_E_ARG, "ADJOINT_CHECKPOINT_BYTES must be >= 0");
-        p->adj_budget = v;
+        if (v != v) return fail(GS_E_ARG, "ADJOINT_CHECKPOINT_BYTES must be a number");
+        p->adj_budget = v < 0 ? -1.0 : v;     // negative: 60 % of the free device memory
     }
     else return fail(GS_E_ARG, "scalar " + n + " is not settable");
     return 0;
@@ -1251,10 +1252,19 @@ int gs_adjoint_forward(gs_plan* p, int n, const double* dz_rk, const double* dz_
         p->adj_lin.clear();
         // one segment when every step's FIELD fits the budget; otherwise segments
         // of K ~ sqrt(2n) steps (memory ceil(n/K) x 2 + K fields, one recomputation)
+        double budget = p->adj_budget;
+        if (budget < 0) {
+            size_t dev_free = 0, dev_total = 0;
+            HCK(hipMemGetInfo(&dev_free, &dev_total));
+            budget = 0.6 * (double)(dev_free + p->ckpt.n);     // the old checkpoints are freed first
+        }
         int K = n;
-        if ((double)n * (double)fb > p->adj_budget) K = std::max(1, (int)std::ceil(std::sqrt(2.0 * n)));
+        if ((double)n * (double)fb > budget) K = std::max(1, (int)std::ceil(std::sqrt(2.0 * n)));
         const int nseg = (n + K - 1) / K;
-        if (p->ckpt.n != fb * (size_t)K) p->ckpt.alloc(fb * (size_t)K);
+        if (p->ckpt.n != fb * (size_t)K) {
+            p->ckpt.alloc(0);
+            p->ckpt.alloc(fb * (size_t)K);
+        }
         if (K < n) {
             if (p->snaps.n != 2 * fb * (size_t)nseg) p->snaps.alloc(2 * fb * (size_t)nseg);
             for (DevBuf* b : {&p->rfield, &p->rb1, &p->rfk})

@@ -155,8 +155,9 @@ int gs_fixed_steps(gs_plan* p, double dz, int n, double* ms4);
  * ZSTEP and the linear step at the new one; scalar "LAST_DZ_NL" after gs_step
  * gives the former, "ZSTEP" the latter).  The state is left at the end of the
  * march.  FIELD is kept on the device before every step when n of them fit in
- * the scalar "ADJOINT_CHECKPOINT_BYTES" (default 16 GiB; gs_set_scalar);
- * otherwise FIELD and BIGBUFFER2 are kept every K = ceil(sqrt(2n)) steps and
+ * the scalar "ADJOINT_CHECKPOINT_BYTES" (gs_set_scalar; default, or negative:
+ * 60 % of the device memory free at the call); otherwise FIELD and FIELD_k
+ * (the live BIGBUFFER2) are kept every K = ceil(sqrt(2n)) steps and
  * gs_adjoint_backward recomputes one segment of K steps at a time (memory
  * (2 ceil(n/K) + K) fields, one extra forward).
  *
