@@ -27,6 +27,9 @@
 #ifndef GS_NL_E32
 #define GS_NL_E32 8    // samples per thread of the FP32/mixed line kernels for N_T >= 1024
 #endif
+#ifndef GS_SCAN_TOL_F32OUT
+#define GS_SCAN_TOL_F32OUT 4e-15   // Newton stop of the FP64 plasma scan when rho is stored as FP32
+#endif
 #ifndef GS_NL_WAVES
 #define GS_NL_WAVES 1  // minimum waves per SIMD the RK4 kernel is register-allocated for
 #endif
@@ -214,7 +217,7 @@ __device__ bool plasma_scan2(const Wt (&Wv)[E], const Wt (&cv)[E], O (&rho)[E], 
     const S Wm1 = tid > 0 ? (S)ps.W[tid - 1] : (S)0, cm1 = tid > 0 ? (S)ps.c[tid - 1] : (S)0;
     const S rho0 = (S)a.rho_init;
     AffR<S>* sc = (AffR<S>*)ps.scan;
-    const S tol = sizeof(S) == 8 ? (S)4e-15 : (S)2e-7;
+    const S tol = sizeof(S) == 8 ? (S)(sizeof(O) == 4 ? GS_SCAN_TOL_F32OUT : 4e-15) : (S)2e-7;
     S g = 0;
     bool done = false;
     for (int it = 0; it < 24 && !done; ++it) {
