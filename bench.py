@@ -186,7 +186,8 @@ def gradient_check(gsa, workloads, work, z_max, ndir, energy, max_steps, seed=20
     second of two, after the checkpoint memory is allocated) per precision
     and, along ndir seeded random phase directions, the relative
     mismatch of each gradient's directional derivative against central
-    differences of the FP64 forward."""
+    differences of the FP64 forward, and the relative L2 distance of the
+    mixed and FP32 gradients from the FP64 one."""
     import numpy as np
     import torch
     from gsolver_amd.adjoint import PhaseMarch, record_steps
@@ -249,9 +250,12 @@ def gradient_check(gsa, workloads, work, z_max, ndir, energy, max_steps, seed=20
     for prec in grads:
         out[prec]["fd_rel_err_max"] = float(np.max(errs[prec]))
         out[prec]["fd_rel_err_median"] = float(np.median(errs[prec]))
+        out[prec]["rel_l2_vs_fp64_gradient"] = float(np.linalg.norm(grads[prec] - grads["fp64"])
+                                                     / np.linalg.norm(grads["fp64"]))
     p64.close()
-    print("[bench] gradient vs central differences (max over %d directions): %s" % (
-        ndir, {k: "%.2e" % out[k]["fd_rel_err_max"] for k in grads}), file=sys.stderr, flush=True)
+    print("[bench] gradient vs central differences (max over %d directions): %s; vs the FP64 gradient: %s" % (
+        ndir, {k: "%.2e" % out[k]["fd_rel_err_max"] for k in grads},
+        {k: "%.2e" % out[k]["rel_l2_vs_fp64_gradient"] for k in ("mixed", "fp32")}), file=sys.stderr, flush=True)
     return out
 
 
