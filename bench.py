@@ -296,6 +296,9 @@ def main():
                     help="pulse energy of the gradient check in J (0.3 uJ: about twice the critical power, "
                          "collapse beyond 1 cm)")
     ap.add_argument("--gradient-max-steps", type=int, default=50000, help="cap on the recorded steps")
+    ap.add_argument("--components", action="store_true",
+                    help="only run the stand-alone line FFT, linear step and single-RHS kernels at 1024 x 4096 "
+                         "(mixed and FP32; for rocprofv3 kernel traces)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -321,6 +324,23 @@ def main():
 
     work = tempfile.mkdtemp(prefix="gs_bench_")
     cfg = workloads.write_config(work, "c1")
+    if args.components:
+        # the path's FFT, pointwise and RHS work as separate kernels (the march fuses them):
+        # batched 4096-point line FFTs over all 1024 lines, the stand-alone linear step,
+        # and one RHS evaluation per line; host copies are outside the kernels
+        out = {}
+        for prec in ("mixed", "fp32"):
+            p = gsa.Plan(cfg, precision=prec)
+            F = p.get()
+            for sign in (-1, 1, -1):
+                p.fft_lines(F, sign)
+            for _ in range(3):
+                p.linear_step(p.scalar("ZSTEP"))
+            p.nl_response(F)
+            out[prec] = {"R": p.R, "T": p.T}
+            p.close()
+        print(json.dumps({"components": out}))
+        return
     probe = gsa.Plan(cfg, precision="fp32")
     dz = probe.scalar("ZSTEP")
     probe.close()
