@@ -41,8 +41,8 @@ class PhaseMarch(torch.autograd.Function):
 
     phi: real tensor [N_T]; field0: complex [N_R, N_T] (r, omega) numpy array;
     returns the complex128 field [N_R, N_T] after the march (CPU tensor).  The
-    plan's state is overwritten.  One backward per forward (the plan keeps the
-    checkpoints of the last forward only)."""
+    plan's state is overwritten.  The plan keeps the checkpoints of its last
+    forward only: a backward after a later forward on the same plan raises."""
 
     @staticmethod
     def forward(ctx, phi, plan, field0, dz_rk, dz_lin):
@@ -52,13 +52,18 @@ class PhaseMarch(torch.autograd.Function):
         plan.put(A0, B2)
         plan.qdht(B2)
         plan.adjoint_forward(dz_rk, dz_lin)
+        plan._adjoint_generation = getattr(plan, "_adjoint_generation", 0) + 1
         ctx.plan = plan
+        ctx.generation = plan._adjoint_generation
         ctx.A0 = A0
         ctx.phi_meta = (phi.dtype, phi.device)
         return torch.from_numpy(plan.get(FIELD))
 
     @staticmethod
     def backward(ctx, grad_field):
+        if getattr(ctx.plan, "_adjoint_generation", None) != ctx.generation:
+            raise RuntimeError("PhaseMarch.backward: the plan has run another forward since this one; "
+                               "its checkpoints belong to that march (use one plan per pending graph)")
         g = grad_field.detach().to("cpu", torch.complex128).numpy()
         g0, _ = ctx.plan.adjoint_backward(g, None, tie_b2=True)
         gphi = np.imag(g0 * np.conj(ctx.A0)).sum(axis=0)

@@ -79,3 +79,50 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_record_steps_needs_exactly_one_stop():
+    """record_steps stops either after n_steps or at z_end: giving both or
+    neither is an error before the plan is touched."""
+    from gsolver_amd.adjoint import record_steps
+    with pytest.raises(ValueError):
+        record_steps(None)
+    with pytest.raises(ValueError):
+        record_steps(None, n_steps=3, z_end=1e-3)
+
+
+def test_phase_march_backward_after_later_forward_raises():
+    """A plan keeps the checkpoints of its last forward only: the backward of an
+    earlier PhaseMarch on the same plan raises (checked with a stand-in plan
+    object; no GPU involved)."""
+    torch = pytest.importorskip("torch")
+    from gsolver_amd.adjoint import PhaseMarch
+
+    class FakePlan:
+        R, T = 2, 4
+
+        def put(self, a, buf):
+            self.a = np.asarray(a)
+
+        def qdht(self, buf):
+            pass
+
+        def adjoint_forward(self, rk, lin):
+            pass
+
+        def get(self, buf):
+            return self.a
+
+        def adjoint_backward(self, g, gb2, tie_b2=True):
+            return np.asarray(g), np.zeros_like(g)
+
+    p = FakePlan()
+    f0 = np.ones((2, 4), dtype=np.complex128)
+    phi1 = torch.zeros(4, dtype=torch.float64, requires_grad=True)
+    out1 = PhaseMarch.apply(phi1, p, f0, [1e-6], [1e-6])
+    phi2 = torch.zeros(4, dtype=torch.float64, requires_grad=True)
+    out2 = PhaseMarch.apply(phi2, p, f0, [1e-6], [1e-6])
+    (out2.abs() ** 2).sum().backward()          # the latest forward: fine
+    assert phi2.grad is not None
+    with pytest.raises(RuntimeError, match="another forward"):
+        (out1.abs() ** 2).sum().backward()
