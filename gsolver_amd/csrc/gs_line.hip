@@ -104,6 +104,37 @@ __global__ __launch_bounds__((Shape<Pol, T, E>::NT)) void k_nl_response(const C2
 }
 
 // ------------------------------------------------------------------ line FFT / diagnostics
+// E consecutive complex values (one chunk) from / to global memory, as 16-byte
+// accesses (the chunk starts on a 16-byte boundary)
+template <class R, int E>
+__device__ __forceinline__ void load_chunk(const C2<R>* __restrict__ p, C2<R> (&x)[E])
+{
+    if constexpr (sizeof(R) == 4) {
+        const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+        for (int i = 0; i < E / 2; ++i) {
+            const float4 v = q[i];
+            x[2 * i] = C2<R>{v.x, v.y};
+            x[2 * i + 1] = C2<R>{v.z, v.w};
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) x[e] = p[e];
+    }
+}
+template <class R, int E>
+__device__ __forceinline__ void store_chunk(C2<R>* __restrict__ p, const C2<R> (&x)[E])
+{
+    if constexpr (sizeof(R) == 4) {
+        float4* q = reinterpret_cast<float4*>(p);
+#pragma unroll
+        for (int i = 0; i < E / 2; ++i) q[i] = float4{x[2 * i].x, x[2 * i].y, x[2 * i + 1].x, x[2 * i + 1].y};
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e) p[e] = x[e];
+    }
+}
+
 template <class R, int T>
 struct FftShape {
     static constexpr int E = T >= 1024 ? 8 : (T >= 512 ? 8 : 4);
@@ -126,12 +157,9 @@ __global__ __launch_bounds__((FftShape<R, T>::NT)) void k_fft_lines(const C2<R>*
     if (sign < 0) {
         // chunk in -> strided out
         R f[E];
+        load_chunk<R, E>(in + base + (size_t)tid * E, x);
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int n = tid * E + e;
-            x[e] = in[base + n];
-            f[e] = (R)(scale * (filt ? filt[n] : 1.0));
-        }
+        for (int e = 0; e < E; ++e) f[e] = (R)(scale * (filt ? filt[tid * E + e] : 1.0));
 #pragma unroll
         for (int r = 0; r < E; ++r) y[r] = C2<R>{0, 0};
         fft_fwd_acc<0, R, T, NT, E>(x, f, y, b0, b1, tw, nullptr);
@@ -145,8 +173,7 @@ __global__ __launch_bounds__((FftShape<R, T>::NT)) void k_fft_lines(const C2<R>*
             y[r] = cscale(in[base + n], (R)(scale * (filt ? filt[n] : 1.0)));
         }
         fft_inv<R, T, NT, E>(y, x, b0, b1, tw);
-#pragma unroll
-        for (int e = 0; e < E; ++e) out[base + tid * E + e] = x[e];
+        store_chunk<R, E>(out + base + (size_t)tid * E, x);
     }
 }
 
