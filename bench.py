@@ -284,7 +284,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4,
                     help="fixed-dz steps of the FP64 oracle timed for cpu_baseline (about 3 s each on 16 threads)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the adaptive-march accuracy check")
-    ap.add_argument("--batch-runs", type=int, default=16, help="runs of the energy/width scan (0: skip)")
+    ap.add_argument("--batch-runs", type=int, default=64, help="runs of the energy/width scan (0: skip)")
     ap.add_argument("--batch-zmax", type=float, default=5e-6, help="distance of each scan run in m")
     ap.add_argument("--accuracy-zmax", type=float, default=1e-3,
                     help="distance of the accuracy-check march in m (two dump planes)")
