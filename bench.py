@@ -290,7 +290,7 @@ def main():
                     help="distance of the accuracy-check march in m (two dump planes)")
     ap.add_argument("--gradient-zmax", type=float, default=1e-2,
                     help="march length of the reverse-mode check at N_r=512 x N_t=2048 (0: skip)")
-    ap.add_argument("--gradient-dirs", type=int, default=8,
+    ap.add_argument("--gradient-dirs", type=int, default=4,
                     help="random directions of the gradient check (each costs two FP64 marches)")
     ap.add_argument("--gradient-energy", type=float, default=0.3e-6,
                     help="pulse energy of the gradient check in J (0.3 uJ: about twice the critical power, "
