@@ -130,6 +130,32 @@ def measured_traffic(kernel):
         return None
 
 
+def measured_valu_insts(kernel):
+    """VALU wave-instructions per launch of `kernel` from the committed
+    rocprofv3 SQ_INSTS_VALU pass, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)["kernels"][kernel]["valu_insts"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+# VALU issue peak: one wave64 instruction per 4 cycles per SIMD (16 lanes),
+# 4 SIMDs per CU, 256 CUs, 2.4 GHz
+VALU_PEAK_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
+
+
+def valu_roofline(kernel, ms):
+    """issue rate of the kernel's VALU wave-instructions (committed SQ count /
+    live average kernel time) against the VALU issue peak"""
+    n = measured_valu_insts(kernel)
+    if n is None or not ms:
+        return None
+    ach = n / (ms * 1e-3)
+    return {"bound": "valu", "achieved": ach, "peak": VALU_PEAK_WAVE_INSTS, "unit": "wave-instr/s",
+            "frac": ach / VALU_PEAK_WAVE_INSTS, "valu_insts_per_launch": n, "kernel": kernel}
+
+
 def batch_check(gsa, workloads, work, nruns, z_max, seed=20261202):
     """Energy / width scan over uneven grids: nruns runs with pulse energy
     log-uniform in 0.5-10 uJ, w0 uniform in 40-150 um, N_R in {128, 256, 512}
@@ -407,6 +433,7 @@ def main():
         "roofline_qdht_fp32": roofline("qdht", res["fp32"]["kernel_ms"]["qdht_back"], R, T, "fp32",
                                        "k_qdht_f32<false>"),
         "roofline_nl": roofline("nl", km["nl_rk4"], R, T, "mixed", "k_nl_rk4<PolMix>"),
+        "roofline_nl_valu": valu_roofline("k_nl_rk4<PolMix>", km["nl_rk4"]),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
