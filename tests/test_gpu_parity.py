@@ -229,6 +229,37 @@ def test_adaptive_march_vs_oracle(tmp_path, prec):
         assert max(errs) < (tol if prec != "fp64" else 1e-9)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_every_dump_type_vs_oracle(tmp_path, prec):
+    """All seven .udf dump types of dump_types.cpp (full E(r,t), maxI, flux,
+    plasma_max, field_axis, average_spectrum, plasma_axis) written by the
+    adaptive march of an ionizing run (peak intensity about 20 in solver units,
+    below the loss gate), against the oracle's files: same headers and slab
+    counts, slabs to the precision's tolerance."""
+    names = ("full", "maxI", "flux", "plasma_max", "field_axis", "average_spectrum", "plasma_axis")
+    kw = dict(HOT, ENERGY="12e-6", N_T="256", N_Z="3", Z_MAX="6e-5", MAX_TOLERANCE="3e-3", DUMPS_N=str(len(names)))
+    for i, n in enumerate(names):
+        kw["DUMPTYPE%d" % i] = n
+        kw["DUMP_FILENAME%d" % i] = "a_%%s_%d.udf" % i
+    cfg = write_cfg(tmp_path / "c.txt", **kw)
+    gdir, odir = tmp_path / "gpu", tmp_path / "orc"
+    gdir.mkdir()
+    odir.mkdir()
+    p = gsa.Plan(cfg, precision=prec)
+    o = oracle.Sim(cfg, nthreads=8)
+    assert p.march(str(gdir)) == 0
+    o.run(str(odir))
+    tol = 1e-9 if prec == "fp64" else 2e-4
+    for i, n in enumerate(names):
+        hg, g = oracle.load_udf(str(gdir / ("a__%d.udf" % i)))
+        ho, r = oracle.load_udf(str(odir / ("a__%d.udf" % i)))
+        assert (hg == ho).all(), n
+        assert g.shape == r.shape, n
+        errs = [rel_l2(g[z], r[z]) if np.linalg.norm(r[z]) > 0 else float(np.abs(g[z]).max()) for z in range(g.shape[0])]
+        print("dump", prec, n, g.shape, max(errs))
+        assert max(errs) < (1e-3 if n in ("plasma_max", "plasma_axis") and prec != "fp64" else tol), (n, errs)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
 def test_device_driven_march_matches_host_controller(tmp_path, prec, monkeypatch):
     """gs_march runs the reference controller on the device (step attempts
