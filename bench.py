@@ -74,12 +74,15 @@ def read_udf(path):
     return data.reshape(zN, -1)
 
 
-def accuracy_check(gsa, workloads, work, z_max, planes):
+def accuracy_check(gsa, workloads, work, z_max, planes, precisions=("fp64", "mixed", "fp32"), max_steps=0,
+                   chunk=20000):
     """Adaptive march (the reference controller, MAX_TOLERANCE 1e-4) of the
     production workload to z_max with `planes` dump planes after z = 0, in
     every precision; fluence (flux dump) and peak plasma density (plasma_max
     dump) at each plane and the field at the last plane, compared with the
-    FP64 GPU run."""
+    FP64 GPU run.  Each march runs as gs_march calls of `chunk` accepted steps
+    (one progress line each); max_steps > 0 stops a precision there, and only
+    the planes every precision reached are compared."""
     import numpy as np
     keys = dict(N_Z=planes + 1, Z_MAX=z_max, DUMPS_N=2, DUMPTYPE0="flux", DUMP_FILENAME0="acc_f.udf",
                 DUMPTYPE1="plasma_max", DUMP_FILENAME1="acc_pm.udf")
@@ -87,36 +90,51 @@ def accuracy_check(gsa, workloads, work, z_max, planes):
     os.makedirs(d, exist_ok=True)
     cfg = workloads.write_config(d, "c1", **keys)
     out = {}
-    for prec in ("fp64", "mixed", "fp32"):
+    for prec in precisions:
         od = os.path.join(d, prec)
         os.makedirs(od, exist_ok=True)
         p = gsa.Plan(cfg, precision=prec)
         t0 = time.perf_counter()
-        p.march(od)
+        done = False
+        while not done:
+            n = chunk if max_steps <= 0 else min(chunk, max_steps - int(p.scalar("STEP_N")))
+            done = p.march(od, n) == 0 or (max_steps > 0 and p.scalar("STEP_N") >= max_steps)
+            print("[bench] accuracy march %s: %d steps, z = %.4g m, dz = %.3g m, %.1f s"
+                  % (prec, p.scalar("STEP_N"), p.scalar("CURRENT_Z"), p.scalar("ZSTEP"), time.perf_counter() - t0),
+                  file=sys.stderr, flush=True)
         wall = time.perf_counter() - t0
+        z_end = p.scalar("CURRENT_Z")
+        znet = list(p.table("znet"))
         out[prec] = {"field": p.get(), "flux": read_udf(os.path.join(od, "acc_f.udf")),
                      "pm": read_udf(os.path.join(od, "acc_pm.udf")), "steps": int(p.scalar("STEP_N")),
-                     "rejected": int(p.scalar("n_reject")), "wall_s": wall, "z": list(p.table("znet"))}
+                     "rejected": int(p.scalar("n_reject")), "wall_s": wall, "z": znet, "z_end": z_end,
+                     "reached": sum(1 for k in range(1, planes + 1) if z_end >= znet[k])}
         p.close()
-        print("[bench] accuracy march %s: %d steps (%d rejected) to z = %.3g m in %.1f s"
-              % (prec, out[prec]["steps"], out[prec]["rejected"], z_max, wall), file=sys.stderr, flush=True)
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b)) if np.linalg.norm(b) > 0 else float(np.abs(a).max())
-    ref = out["fp64"]
-    res = {"z_planes_m": ref["z"][1:], "steps": {k: v["steps"] for k, v in out.items()},
+    first = out[precisions[0]]
+    res = {"z_planes_m": first["z"][1:], "steps": {k: v["steps"] for k, v in out.items()},
            "rejected": {k: v["rejected"] for k, v in out.items()},
            "adaptive_steps_per_s": {k: v["steps"] / v["wall_s"] for k, v in out.items()}}
-    for prec in ("mixed", "fp32"):
+    if max_steps > 0:
+        res["max_steps"] = max_steps
+        res["z_reached_m"] = {k: v["z_end"] for k, v in out.items()}
+    if "fp64" not in out:
+        return res
+    ref = out["fp64"]
+    common = min(v["reached"] for v in out.values())
+    for prec in (q for q in ("mixed", "fp32") if q in out):
         o = out[prec]
         res[prec] = {
-            "field_rel_l2_last_plane": rel(o["field"], ref["field"]),
-            "fluence_rel_l2": [rel(o["flux"][k], ref["flux"][k]) for k in range(1, planes + 1)],
+            "field_rel_l2_last_plane": rel(o["field"], ref["field"]) if common == planes else None,
+            "fluence_rel_l2": [rel(o["flux"][k], ref["flux"][k]) for k in range(1, common + 1)],
             "peak_plasma_rel": [abs(float(o["pm"][k].max()) - float(ref["pm"][k].max())) / float(ref["pm"][k].max())
-                                if ref["pm"][k].max() > 0 else 0.0 for k in range(1, planes + 1)],
+                                if ref["pm"][k].max() > 0 else 0.0 for k in range(1, common + 1)],
         }
-    res["peak_plasma_fp64"] = [float(ref["pm"][k].max()) for k in range(1, planes + 1)]
+    res["peak_plasma_fp64"] = [float(ref["pm"][k].max()) for k in range(1, common + 1)]
     return res
 
 
+WORKLOAD = "N_r=1024 x N_t=4096, R_MAX 1 mm, 800 fs window, fused silica, 5 uJ 40 fs w0=100 um"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_1024x4096.json")
 
 
@@ -313,7 +331,13 @@ def main():
     ap.add_argument("--batch-runs", type=int, default=64, help="runs of the energy/width scan (0: skip)")
     ap.add_argument("--batch-zmax", type=float, default=5e-6, help="distance of each scan run in m")
     ap.add_argument("--accuracy-zmax", type=float, default=1e-3,
-                    help="distance of the accuracy-check march in m (two dump planes)")
+                    help="distance of the accuracy-check march in m")
+    ap.add_argument("--accuracy-planes", type=int, default=2, help="dump planes of the accuracy check after z = 0")
+    ap.add_argument("--accuracy-max-steps", type=int, default=0,
+                    help="cap on the accepted steps of each accuracy march (0: none)")
+    ap.add_argument("--accuracy-only", default=None, metavar="PRECISIONS",
+                    help="only run the accuracy march in these comma-separated precisions (e.g. fp64,mixed,fp32) "
+                         "and print its result")
     ap.add_argument("--gradient-zmax", type=float, default=1e-2,
                     help="march length of the reverse-mode check at N_r=512 x N_t=2048 (0: skip)")
     ap.add_argument("--gradient-dirs", type=int, default=4,
@@ -350,6 +374,12 @@ def main():
 
     work = tempfile.mkdtemp(prefix="gs_bench_")
     cfg = workloads.write_config(work, "c1")
+    if args.accuracy_only:
+        precs = tuple(args.accuracy_only.split(","))
+        acc = accuracy_check(gsa, workloads, work, args.accuracy_zmax, args.accuracy_planes, precs,
+                             args.accuracy_max_steps)
+        print(json.dumps({"accuracy_march": acc, "config": {"workload": WORKLOAD, "z_max_m": args.accuracy_zmax}}))
+        return
     if args.components:
         # the path's FFT, pointwise and RHS work as separate kernels (the march fuses them):
         # batched 4096-point line FFTs over all 1024 lines, the stand-alone linear step,
@@ -393,7 +423,8 @@ def main():
 
     acc = None
     if not args.no_accuracy and not args.no_fp64:
-        acc = accuracy_check(gsa, workloads, work, args.accuracy_zmax, 2)
+        acc = accuracy_check(gsa, workloads, work, args.accuracy_zmax, args.accuracy_planes,
+                             max_steps=args.accuracy_max_steps)
 
     batch = batch_check(gsa, workloads, work, args.batch_runs, args.batch_zmax) if args.batch_runs > 0 else None
 
@@ -418,7 +449,7 @@ def main():
         "vs_baseline": None,
         "dtype": "mixed: bf16x3-mfma/f32-acc QDHT, f32 FFT, f64 plasma scan",
         "data": "synthetic (config-built analytic pulse, seeded 1% amplitude noise)",
-        "config": {"workload": "N_r=1024 x N_t=4096, R_MAX 1 mm, 800 fs window, fused silica, 5 uJ 40 fs w0=100 um",
+        "config": {"workload": WORKLOAD,
                    "grid": [R, T], "dz_m": dz, "mode": "fixed-dz", "precision": "mixed"},
         "fp32_value": res["fp32"]["steps_per_s"],
         "mixed_over_fp32": m["steps_per_s"] / res["fp32"]["steps_per_s"],

@@ -111,6 +111,7 @@ struct gs_plan {
     double ZSTEP = 0, CURRENT_Z = 0;
     long STEP_N = 0, n_reject = 0;
     int n_Z = 0;
+    bool dumps_started = false;   // a march call of this plan has written a dump plane
     double timings[4] = {0, 0, 0, 0};
 
     // device-driven march: controller state, its pinned host mirror, and the
@@ -630,13 +631,23 @@ struct Udf {
     {
         if (f) std::fclose(f);
     }
-    void open(const std::string& path, int iscomplex, int tN, int xN, int yN, int zN)
+    // resume: continue a file of the same layout written by an earlier call
+    // (its header and planes are kept; writes are positioned per plane)
+    void open(const std::string& path, int iscomplex, int tN, int xN, int yN, int zN, bool resume)
     {
+        cplx = iscomplex;
+        const int h[6] = {8, iscomplex, tN, xN, yN, zN};
+        if (resume && (f = std::fopen(path.c_str(), "r+b")) != nullptr) {
+            char tag0[7];
+            int h0[6];
+            if (std::fread(tag0, 1, 7, f) == 7 && std::fread(h0, sizeof(int), 6, f) == 6 &&
+                std::memcmp(tag0, "UDF1.0", 7) == 0 && std::memcmp(h0, h, sizeof h) == 0)
+                return;
+            std::fclose(f);   // another layout: start the file again
+        }
         f = std::fopen(path.c_str(), "w+b");
         if (!f) throw gs::Error(GS_E_CONFIG, "void dump_type_empty_class :: init_dumpfile : Unable to open file!");
-        cplx = iscomplex;
         const char tag[] = "UDF1.0";
-        const int h[6] = {8, iscomplex, tN, xN, yN, zN};
         std::fwrite(tag, 1, sizeof tag, f);
         std::fwrite(h, sizeof(int), 6, f);
         std::fflush(f);
@@ -657,14 +668,15 @@ struct DumpSet {
 void open_dumps(gs_plan* p, const std::string& outdir, DumpSet* ds)
 {
     const auto& s = p->s;
+    const bool resume = p->dumps_started;   // continue the planes an earlier call wrote
     ds->files.resize(s.dumps.size());
     for (size_t i = 0; i < s.dumps.size(); ++i) {
         const auto& d = s.dumps[i];
         const std::string path = outdir + "/" + d.filename;
-        if (d.type == "full") ds->files[i].open(path, 1, s.NT, s.NR, 1, s.NZ / d.nz_denom);
-        else if (d.type == "maxI" || d.type == "flux" || d.type == "plasma_max") ds->files[i].open(path, 0, 1, s.NR, 1, s.NZ);
-        else if (d.type == "field_axis") ds->files[i].open(path, 1, s.NT, 1, 1, s.NZ);
-        else ds->files[i].open(path, 0, s.NT, 1, 1, s.NZ);   // average_spectrum, plasma_axis
+        if (d.type == "full") ds->files[i].open(path, 1, s.NT, s.NR, 1, s.NZ / d.nz_denom, resume);
+        else if (d.type == "maxI" || d.type == "flux" || d.type == "plasma_max") ds->files[i].open(path, 0, 1, s.NR, 1, s.NZ, resume);
+        else if (d.type == "field_axis") ds->files[i].open(path, 1, s.NT, 1, 1, s.NZ, resume);
+        else ds->files[i].open(path, 0, s.NT, 1, 1, s.NZ, resume);   // average_spectrum, plasma_axis
     }
 }
 
@@ -672,6 +684,7 @@ void do_dumps(gs_plan* p, DumpSet* ds)
 {
     const auto& s = p->s;
     const int nr = s.NR, nt = s.NT, nz = p->n_Z;
+    p->dumps_started = true;
     double* dflux = p->diag.as<double>();
     double* dmax = dflux + p->rp;
     double* drho = dmax + p->rp;
@@ -835,7 +848,10 @@ int gs_set_scalar(gs_plan* p, const char* name, double v)
     const std::string n(name);
     if (n == "ZSTEP") p->ZSTEP = v;
     else if (n == "CURRENT_Z") p->CURRENT_Z = v;
-    else if (n == "n_Z") p->n_Z = (int)v;
+    else if (n == "n_Z") {
+        p->n_Z = (int)v;
+        if (p->n_Z == 0) p->dumps_started = false;   // a march from plane 0 starts its files again
+    }
     else if (n == "MAX_TOLERANCE") p->s.MAX_TOLERANCE = v;
     else if (n == "ADJOINT_CHECKPOINT_BYTES") {
         if (v != v) return fail(GS_E_ARG, "ADJOINT_CHECKPOINT_BYTES must be a number");

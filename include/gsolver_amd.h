@@ -119,7 +119,11 @@ int gs_fixed_step(gs_plan* p, double dz, double* maxA2, double* maxNL2);
 /* the main loop of solver_main.cpp:202-225 with the .udf dumps of the config
  * (dump_types.cpp; types flux, maxI, field_axis, plasma_max, average_spectrum,
  * full, plasma_axis) written into outdir.  Stops after max_steps accepted
- * steps when max_steps > 0 (returns 1 then; 0 at the last plane). */
+ * steps when max_steps > 0 (returns 1 then; 0 at the last plane).  A call
+ * after such a stop continues the march; dump files already in outdir keep
+ * the planes written so far (they are reopened, not truncated, once the plan
+ * has written its first plane), so a march run in chunks leaves the same
+ * files as one uninterrupted call. */
 int gs_march(gs_plan* p, const char* outdir, long max_steps);
 
 /* gs_march for n independent runs at once (energy / width scans, uneven

@@ -295,6 +295,36 @@ def test_device_driven_march_matches_host_controller(tmp_path, prec, monkeypatch
         np.testing.assert_array_equal(h[k], g[k])
 
 
+def test_chunked_march_equals_single_call(tmp_path):
+    """A march run as several gs_march calls of at most 25 accepted steps into
+    the same directory leaves the same dump files (every plane, headers
+    included) and the same final field as one uninterrupted call."""
+    kw = dict(HOT, N_Z="4", Z_MAX="1e-4", MAX_TOLERANCE="3e-3", DUMPS_N="2", DUMPTYPE0="flux",
+              DUMP_FILENAME0="c_%s_f.udf", DUMPTYPE1="field_axis", DUMP_FILENAME1="c_%s_fa.udf")
+    cfg = write_cfg(tmp_path / "c.txt", **kw)
+    out = {}
+    for mode in ("single", "chunked"):
+        d = tmp_path / mode
+        d.mkdir()
+        p = gsa.Plan(cfg, precision="mixed")
+        calls = 1
+        if mode == "single":
+            assert p.march(str(d)) == 0
+        else:
+            while p.march(str(d), 25) == 1:
+                calls += 1
+        out[mode] = dict(field=p.get(), steps=p.scalar("STEP_N"), calls=calls,
+                         raw=[(d / n).read_bytes() for n in ("c__f.udf", "c__fa.udf")])
+        p.close()
+    s, c = out["single"], out["chunked"]
+    print("chunked march", s["steps"], c["steps"], c["calls"])
+    assert c["calls"] > 3 and s["steps"] == c["steps"]
+    np.testing.assert_array_equal(s["field"], c["field"])
+    assert s["raw"] == c["raw"]
+    h, f = oracle.load_udf(str(tmp_path / "chunked" / "c__f.udf"))
+    assert f.shape[0] == 4 and all(np.abs(f[k]).max() > 0 for k in range(4))
+
+
 def test_march_many_matches_single_runs(tmp_path):
     """gs_march_many over runs with uneven grids (N_R, N_T) and pulse energies,
     mixed precision: every run's dumps and final field equal those of the same
