@@ -78,11 +78,12 @@ def accuracy_check(gsa, workloads, work, z_max, planes, precisions=("fp64", "mix
                    chunk=20000):
     """Adaptive march (the reference controller, MAX_TOLERANCE 1e-4) of the
     production workload to z_max with `planes` dump planes after z = 0, in
-    every precision; fluence (flux dump) and peak plasma density (plasma_max
-    dump) at each plane and the field at the last plane, compared with the
-    FP64 GPU run.  Each march runs as gs_march calls of `chunk` accepted steps
-    (one progress line each); max_steps > 0 stops a precision there, and only
-    the planes every precision reached are compared."""
+    every precision; fluence (flux dump) and plasma density (plasma_max dump:
+    max_t rho(r) as a profile in relative L2, and its peak) at each plane and
+    the field at the last plane, compared with the FP64 GPU run.  Each march
+    runs as gs_march calls of `chunk` accepted steps (one progress line
+    each); max_steps > 0 stops a precision there, and only the planes every
+    precision reached are compared."""
     import numpy as np
     keys = dict(N_Z=planes + 1, Z_MAX=z_max, DUMPS_N=2, DUMPTYPE0="flux", DUMP_FILENAME0="acc_f.udf",
                 DUMPTYPE1="plasma_max", DUMP_FILENAME1="acc_pm.udf")
@@ -129,6 +130,7 @@ def accuracy_check(gsa, workloads, work, z_max, planes, precisions=("fp64", "mix
             "fluence_rel_l2": [rel(o["flux"][k], ref["flux"][k]) for k in range(1, common + 1)],
             "peak_plasma_rel": [abs(float(o["pm"][k].max()) - float(ref["pm"][k].max())) / float(ref["pm"][k].max())
                                 if ref["pm"][k].max() > 0 else 0.0 for k in range(1, common + 1)],
+            "plasma_max_rel_l2": [rel(o["pm"][k], ref["pm"][k]) for k in range(1, common + 1)],
         }
     res["peak_plasma_fp64"] = [float(ref["pm"][k].max()) for k in range(1, common + 1)]
     return res
