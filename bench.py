@@ -221,10 +221,15 @@ def batch_check(gsa, workloads, work, nruns, z_max, seed=20261202):
     return out
 
 
-def gradient_check(gsa, workloads, work, z_max, ndir, energy, max_steps, seed=20261203):
+BAND_NM = (500.0, 600.0)
+
+
+def gradient_check(gsa, workloads, work, z_max, ndir, energy, max_steps, objective_name="e4", seed=20261203):
     """Reverse mode at N_r=512 x N_t=2048 (0.5 mm / 400 fs window, fused
     silica, 50 fs w0=60 um pulse of the given energy): the gradient of
-    L = sum_t |E(r=0, t)|^4 at the end of a z_max march with respect to the
+    L = sum_t |E(r=0, t)|^4 (objective "e4") or of the on-axis spectral energy
+    L = sum_{omega in 500-600 nm} |FIELD(r=0, omega)|^2 (objective "band") at
+    the end of a z_max march with respect to the
     2048 spectral phases of the input pulse, through torch.autograd
     (PhaseMarch: gs_adjoint_forward / _backward), on the step sizes the
     reference controller chose (recorded once in FP64; at most max_steps).
@@ -246,16 +251,26 @@ def gradient_check(gsa, workloads, work, z_max, ndir, energy, max_steps, seed=20
     rec.close()
     p0 = gsa.Plan(cfg, precision="fp64")     # the config's initial field (rec is at z_end)
     field0 = p0.get()
+    w = p0.table("omega")
     p0.close()
     n = len(dz_rk)
     print("[bench] gradient: %d recorded steps to z=%g m (%.1f s)" % (n, z_end, t_rec), file=sys.stderr, flush=True)
 
-    def objective(field):
-        return (torch.fft.ifft(field[0]).abs() ** 4).sum()
+    c0 = 299792458.0
+    band = torch.tensor((w >= 2 * np.pi * c0 / (BAND_NM[1] * 1e-9)) & (w <= 2 * np.pi * c0 / (BAND_NM[0] * 1e-9)))
+
+    if objective_name == "band":
+        def objective(field):
+            return (field[0][band].abs() ** 2).sum()
+        desc = "sum_{omega in %g-%g nm} |FIELD(0, omega)|^2 (%d bins)" % (BAND_NM[0], BAND_NM[1], int(band.sum()))
+    else:
+        def objective(field):
+            return (torch.fft.ifft(field[0]).abs() ** 4).sum()
+        desc = "sum_t |E(0,t)|^4"
 
     out = {"workload": "N_r=512 x N_t=2048, 0.5 mm, 400 fs, fused silica, %g uJ 50 fs w0=60 um, z %g m"
                        % (energy * 1e6, z_end),
-           "params": int(field0.shape[1]), "steps": n, "z_m": z_end, "objective": "sum_t |E(0,t)|^4",
+           "params": int(field0.shape[1]), "steps": n, "z_m": z_end, "objective": desc,
            "directions": ndir}
     grads, plans = {}, {}
     for prec in ("mixed", "fp32", "fp64"):
@@ -273,6 +288,10 @@ def gradient_check(gsa, workloads, work, z_max, ndir, energy, max_steps, seed=20
                 print("[bench] gradient %s: first call %.3f s (checkpoint allocation included)" % (prec, t2 - t0),
                       file=sys.stderr, flush=True)
         grads[prec] = phi.grad.numpy().copy()
+        if prec == "fp64" and objective_name == "band":
+            with torch.no_grad():
+                f = PhaseMarch.apply(torch.zeros(p.T, dtype=torch.float64), p, field0, dz_rk, dz_lin)
+                out["band_energy_fraction_on_axis"] = float(objective(f) / (f[0].abs() ** 2).sum())
         out[prec] = {"forward_s": t1 - t0, "backward_s": t2 - t1, "gradient_s": t2 - t0,
                      "backward_over_forward": (t2 - t1) / (t1 - t0)}
         if prec != "fp64":
@@ -348,6 +367,9 @@ def main():
                     help="pulse energy of the gradient check in J (0.3 uJ: about twice the critical power, "
                          "collapse beyond 1 cm)")
     ap.add_argument("--gradient-max-steps", type=int, default=50000, help="cap on the recorded steps")
+    ap.add_argument("--gradient-objective", choices=("e4", "band"), default="e4",
+                    help="e4: sum_t |E(0,t)|^4; band: on-axis spectral energy in 500-600 nm")
+    ap.add_argument("--gradient-only", action="store_true", help="only run the reverse-mode check")
     ap.add_argument("--components", action="store_true",
                     help="only run the stand-alone line FFT, linear step and single-RHS kernels at 1024 x 4096 "
                          "(mixed and FP32; for rocprofv3 kernel traces)")
@@ -376,6 +398,11 @@ def main():
 
     work = tempfile.mkdtemp(prefix="gs_bench_")
     cfg = workloads.write_config(work, "c1")
+    if args.gradient_only:
+        grad = gradient_check(gsa, workloads, work, args.gradient_zmax, args.gradient_dirs, args.gradient_energy,
+                              args.gradient_max_steps, args.gradient_objective)
+        print(json.dumps({"phase_gradient": grad}))
+        return
     if args.accuracy_only:
         precs = tuple(args.accuracy_only.split(","))
         acc = accuracy_check(gsa, workloads, work, args.accuracy_zmax, args.accuracy_planes, precs,
@@ -431,7 +458,7 @@ def main():
     batch = batch_check(gsa, workloads, work, args.batch_runs, args.batch_zmax) if args.batch_runs > 0 else None
 
     grad = (gradient_check(gsa, workloads, work, args.gradient_zmax, args.gradient_dirs, args.gradient_energy,
-                           args.gradient_max_steps) if args.gradient_zmax > 0 else None)
+                           args.gradient_max_steps, args.gradient_objective) if args.gradient_zmax > 0 else None)
 
     cpu = None
     if not args.no_cpu_baseline:
