@@ -417,3 +417,14 @@ def test_step_collapse_error_matches_reference_message(tmp_path, monkeypatch, dr
     o = oracle.Sim(cfg, nthreads=4)
     with pytest.raises(oracle.OracleError, match="Minimum Z step size reached"):
         o.propagator()
+
+
+@pytest.mark.parametrize("nt", ["128", "3000", "8192"])
+def test_unsupported_line_length_is_rejected(tmp_path, nt):
+    """Line lengths outside the kernels' shapes (powers of two 256..4096) stop
+    plan creation with GS_E_ARG before any device work; nothing is rounded or
+    padded behind the caller's back."""
+    cfg = write_cfg(tmp_path / "c.txt", **dict(HOT, N_T=nt))
+    with pytest.raises(gsa.GsError, match="N_T must be a power of two between 256 and 4096") as ei:
+        gsa.Plan(cfg, precision="mixed")
+    assert ei.value.code == -1

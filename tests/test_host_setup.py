@@ -126,3 +126,12 @@ def test_phase_march_backward_after_later_forward_raises():
     assert phi2.grad is not None
     with pytest.raises(RuntimeError, match="another forward"):
         (out1.abs() ** 2).sum().backward()
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is visible here")
+def test_plan_without_gpu_fails_loudly(tmp_path):
+    """No CPU fallback: without a visible GPU, creating a plan raises (after
+    the host set-up has parsed the config) instead of computing anything."""
+    cfg = write_cfg(tmp_path / "c.txt", N_R=64, N_T=256)
+    with pytest.raises(gsa.GsError, match="no GPU visible"):
+        gsa.Plan(cfg, precision="mixed")
