@@ -65,6 +65,26 @@ def cpu_baseline(cfg, dz, nsteps, threads):
                       % (nsteps, threads, dt)}
 
 
+def cpu_baseline_march(cfg, nsteps, threads, full_steps=None):
+    """cpu_baseline of an adaptive march: nsteps accepted steps of the FP64
+    oracle's controller from the workload's initial field (the oracle's
+    propagator(), the reference's per-step work), and the wall time the whole
+    march would take at that rate when full_steps (the GPU run's step count)
+    is given."""
+    import oracle
+    sim = oracle.Sim(cfg, nthreads=threads, precision="fp64")
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        sim.propagator()
+    dt = time.perf_counter() - t0
+    sim.close()
+    out = {"value": nsteps / dt, "unit": "z-steps/s", "cores": threads, "kind": "port",
+           "sample": "%d adaptive z-step(s) of the FP64 oracle from z = 0, %d threads, %.1f s" % (nsteps, threads, dt)}
+    if full_steps:
+        out["projected_full_march_s"] = full_steps * dt / nsteps
+    return out
+
+
 def read_udf(path):
     """A .udf dump (dump_types.cpp:25-72): "UDF1.0\\0", 6 int32 header, float64 data."""
     import numpy as np
@@ -75,26 +95,36 @@ def read_udf(path):
 
 
 def accuracy_check(gsa, workloads, work, z_max, planes, precisions=("fp64", "mixed", "fp32"), max_steps=0,
-                   chunk=20000):
+                   chunk=20000, workload="c1", perturb=1e-12):
     """Adaptive march (the reference controller, MAX_TOLERANCE 1e-4) of the
-    production workload to z_max with `planes` dump planes after z = 0, in
+    production workload (or another named workload) to z_max with `planes`
+    dump planes after z = 0, in
     every precision; fluence (flux dump) and plasma density (plasma_max dump:
     max_t rho(r) as a profile in relative L2, and its peak) at each plane and
     the field at the last plane, compared with the FP64 GPU run.  Each march
     runs as gs_march calls of `chunk` accepted steps (one progress line
     each); max_steps > 0 stops a precision there, and only the planes every
-    precision reached are compared."""
+    precision reached are compared.  The precision "fp64_perturbed" is the
+    FP64 march from an initial state with relative noise `perturb`: how far a
+    perturbation at the rounding level grows over the same march."""
     import numpy as np
     keys = dict(N_Z=planes + 1, Z_MAX=z_max, DUMPS_N=2, DUMPTYPE0="flux", DUMP_FILENAME0="acc_f.udf",
                 DUMPTYPE1="plasma_max", DUMP_FILENAME1="acc_pm.udf")
     d = os.path.join(work, "acc")
     os.makedirs(d, exist_ok=True)
-    cfg = workloads.write_config(d, "c1", **keys)
+    cfg = workloads.write_config(d, workload, **keys)
     out = {}
     for prec in precisions:
         od = os.path.join(d, prec)
         os.makedirs(od, exist_ok=True)
-        p = gsa.Plan(cfg, precision=prec)
+        p = gsa.Plan(cfg, precision="fp64" if prec == "fp64_perturbed" else prec)
+        if prec == "fp64_perturbed":
+            # sensitivity probe: the FP64 march from FIELD and BIGBUFFER2 with seeded
+            # relative noise of size `perturb` (the march's state at z = 0)
+            rng = np.random.default_rng(4242)
+            for buf in (gsa.FIELD, gsa.B2):
+                f = p.get(buf)
+                p.put(f * (1 + perturb * rng.standard_normal(f.shape)), buf)
         t0 = time.perf_counter()
         done = False
         while not done:
@@ -123,7 +153,7 @@ def accuracy_check(gsa, workloads, work, z_max, planes, precisions=("fp64", "mix
         return res
     ref = out["fp64"]
     common = min(v["reached"] for v in out.values())
-    for prec in (q for q in ("mixed", "fp32") if q in out):
+    for prec in (q for q in ("mixed", "fp32", "fp64_perturbed") if q in out):
         o = out[prec]
         res[prec] = {
             "field_rel_l2_last_plane": rel(o["field"], ref["field"]) if common == planes else None,
@@ -137,6 +167,7 @@ def accuracy_check(gsa, workloads, work, z_max, planes, precisions=("fp64", "mix
 
 
 WORKLOAD = "N_r=1024 x N_t=4096, R_MAX 1 mm, 800 fs window, fused silica, 5 uJ 40 fs w0=100 um"
+WORKLOAD_C0 = "N_r=256 x N_t=1024, R_MAX 0.5 mm, 400 fs window, fused silica, 1 uJ 50 fs w0=60 um"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_1024x4096.json")
 
 
@@ -356,9 +387,14 @@ def main():
     ap.add_argument("--accuracy-planes", type=int, default=2, help="dump planes of the accuracy check after z = 0")
     ap.add_argument("--accuracy-max-steps", type=int, default=0,
                     help="cap on the accepted steps of each accuracy march (0: none)")
+    ap.add_argument("--accuracy-workload", default="c1", choices=("c0", "c1"),
+                    help="c1: 1024 x 4096 production workload; c0: 256 x 1024, 1 uJ over 2 cm")
+    ap.add_argument("--accuracy-chunk", type=int, default=20000, help="accepted steps per gs_march call")
+    ap.add_argument("--accuracy-cpu-steps", type=int, default=0,
+                    help="with --accuracy-only: also time this many adaptive steps of the FP64 oracle (cpu_baseline)")
     ap.add_argument("--accuracy-only", default=None, metavar="PRECISIONS",
-                    help="only run the accuracy march in these comma-separated precisions (e.g. fp64,mixed,fp32) "
-                         "and print its result")
+                    help="only run the accuracy march in these comma-separated precisions (e.g. fp64,mixed,fp32; "
+                         "fp64_perturbed: FP64 from a state with 1e-12 relative noise) and print its result")
     ap.add_argument("--gradient-zmax", type=float, default=1e-2,
                     help="march length of the reverse-mode check at N_r=512 x N_t=2048 (0: skip)")
     ap.add_argument("--gradient-dirs", type=int, default=4,
@@ -406,8 +442,17 @@ def main():
     if args.accuracy_only:
         precs = tuple(args.accuracy_only.split(","))
         acc = accuracy_check(gsa, workloads, work, args.accuracy_zmax, args.accuracy_planes, precs,
-                             args.accuracy_max_steps)
-        print(json.dumps({"accuracy_march": acc, "config": {"workload": WORKLOAD, "z_max_m": args.accuracy_zmax}}))
+                             args.accuracy_max_steps, args.accuracy_chunk, args.accuracy_workload)
+        wl = WORKLOAD if args.accuracy_workload == "c1" else WORKLOAD_C0
+        cpu = None
+        if args.accuracy_cpu_steps > 0:
+            threads = max(1, min(16, os.cpu_count() or 1))
+            ccfg = workloads.write_config(os.path.join(work, "cpu"), args.accuracy_workload, N_Z=2,
+                                          Z_MAX=args.accuracy_zmax)
+            cpu = cpu_baseline_march(ccfg, args.accuracy_cpu_steps, threads, max(acc["steps"].values()))
+            print("[bench] cpu_baseline: %s" % cpu, file=sys.stderr, flush=True)
+        print(json.dumps({"accuracy_march": acc, "config": {"workload": wl, "z_max_m": args.accuracy_zmax},
+                          "cpu_baseline": cpu}))
         return
     if args.components:
         # the path's FFT, pointwise and RHS work as separate kernels (the march fuses them):

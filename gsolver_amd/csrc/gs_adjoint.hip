@@ -454,9 +454,10 @@ __global__ void k_lstep_adj(const C2<typename Pol::R>* __restrict__ gfk, C2<type
         gb1k[o] = C2<R>{0, 0};
         return;
     }
-    R sn, cs;
-    lstep_phasor<Pol>((P)la.dz * ((const P*)la.rate)[o], &sn, &cs);
-    const R amp = lstep_amp<Pol>((int)(o % nt), la);
+    P sp, cp;
+    lstep_phasor<Pol>((P)la.dz * ((const P*)la.rate)[o], &sp, &cp);
+    const R sn = (R)sp, cs = (R)cp;
+    const R amp = (R)lstep_amp<Pol>((int)(o % nt), la);
     const C2<R> v = cadd(gb2[o], gfk[o]);
     const C2<R> u = {amp * (cs * v.x + sn * v.y), amp * (cs * v.y - sn * v.x)};
     gb2[o] = u;

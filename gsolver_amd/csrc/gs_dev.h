@@ -320,13 +320,13 @@ __device__ __forceinline__ S heun_step(S x, S Wp, S cp, S Wn, S cn, S dt, S N, S
 }
 
 // ------------------------------------------------------------------ linear step
-// e^{i ph} for the linear-step phase.  MIXED reduces ph to r in [-pi/4, pi/4]
-// and a quadrant q in FP64 (pi/2 split in two doubles), then evaluates sin r
-// and cos r in FP32 from their Taylor series (terms to r^9 and r^10: the
-// first omitted terms are below 2e-9 on the interval) and rotates by q.
-// FP64 and FP32 use sincos / sincosf in their own precision.
+// e^{i ph} for the linear-step phase, in the policy's phase precision P.
+// MIXED reduces ph to r in [-pi/4, pi/4] and a quadrant q (pi/2 split in two
+// doubles) and evaluates sin r and cos r from their Taylor series in FP64
+// (terms to r^13 and r^14: the first omitted terms are below 3e-14 on the
+// interval), then rotates by q.  FP64 and FP32 use sincos / sincosf.
 template <class Pol>
-__device__ __forceinline__ void lstep_phasor(typename Pol::P ph, typename Pol::R* sn, typename Pol::R* cs)
+__device__ __forceinline__ void lstep_phasor(typename Pol::P ph, typename Pol::P* sn, typename Pol::P* cs)
 {
     using R = typename Pol::R;
     using P = typename Pol::P;
@@ -336,21 +336,86 @@ __device__ __forceinline__ void lstep_phasor(typename Pol::P ph, typename Pol::R
         sincosf(ph, sn, cs);
     } else {
         const double q = rint(ph * 0.63661977236758134);            // 2/pi
-        double rd = fma(-q, 1.5707963267948966, ph);                 // pi/2, high part
-        rd = fma(-q, 6.123233995736766e-17, rd);                     // pi/2, low part
-        const float r = (float)rd, r2 = r * r;
-        const float s = r * (1.f + r2 * (-1.f / 6 + r2 * (1.f / 120 + r2 * (-1.f / 5040 + r2 * (1.f / 362880)))));
-        const float c = 1.f + r2 * (-0.5f + r2 * (1.f / 24 + r2 * (-1.f / 720 + r2 * (1.f / 40320 + r2 * (-1.f / 3628800)))));
+        double r = fma(-q, 1.5707963267948966, ph);                  // pi/2, high part
+        r = fma(-q, 6.123233995736766e-17, r);                       // pi/2, low part
+        const double r2 = r * r;
+        double sp = 1.0 / 6227020800.0, cp = 1.0 / 87178291200.0;          // 1/13!, 1/14!
+        sp = fma(sp, -r2, 1.0 / 39916800.0);                                // Horner from the top
+        sp = fma(sp, -r2, 1.0 / 362880.0);
+        sp = fma(sp, -r2, 1.0 / 5040.0);
+        sp = fma(sp, -r2, 1.0 / 120.0);
+        sp = fma(sp, -r2, 1.0 / 6.0);
+        const double sv = fma(-r * r2, sp, r);
+        cp = fma(cp, -r2, 1.0 / 479001600.0);
+        cp = fma(cp, -r2, 1.0 / 3628800.0);
+        cp = fma(cp, -r2, 1.0 / 40320.0);
+        cp = fma(cp, -r2, 1.0 / 720.0);
+        cp = fma(cp, -r2, 1.0 / 24.0);
+        cp = fma(cp, -r2, 0.5);
+        const double cv = fma(-r2, cp, 1.0);
         const int iq = (int)(((long long)q) & 3);
-        *sn = iq == 0 ? s : (iq == 1 ? c : (iq == 2 ? -s : -c));
-        *cs = iq == 0 ? c : (iq == 1 ? -s : (iq == 2 ? -c : s));
+        *sn = iq == 0 ? sv : (iq == 1 ? cv : (iq == 2 ? -sv : -cv));
+        *cs = iq == 0 ? cv : (iq == 1 ? -sv : (iq == 2 ? -cv : sv));
     }
 }
 
+// 32 well-mixed bits from a point index and the bits of its inputs (the
+// finaliser of MurmurHash3); deterministic, so reruns are bit-identical
+__device__ __forceinline__ uint32_t mix_bits(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// x rounded to one of its two FP32 neighbours, the farther one with
+// probability |x - nearest| / (neighbour spacing): the low 29 bits of u are
+// added to the 29 FP64 mantissa bits below FP32 precision and those bits are
+// then cleared (truncation of the magnitude), so the rounding error has zero
+// mean.  A float state updated by steps whose increments fall below half an
+// ulp (round to nearest drops them every step) then drifts as a random walk
+// instead of coherently.  Exact for every x in the FP32 normal range.
+__device__ __forceinline__ float round_unbiased(double x, uint32_t u)
+{
+    unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    b = (b + (u & 0x1fffffffu)) & ~0x1fffffffull;
+    return (float)__longlong_as_double((long long)b);
+}
+
 // one point of calculate_Lstep: B2 += dz B1k g; FIELD = B2 e^{i dz rate} amp; B2 = FIELD,
-// with amp = e^{dz Im k(omega)} of the point's column; B2 lives in fk (LstepArgs)
+// with amp = e^{dz Im k(omega)} of the point's column; B2 lives in fk (LstepArgs).
+// MIXED (FP32 state, FP64 phase) updates in FP64 and stores the state with
+// unbiased rounding: at small dz the per-step B1 increment is below half an
+// FP32 ulp of B2 on most points, and round-to-nearest would drop it every step.
 template <class Pol>
-__device__ __forceinline__ void lstep_point(size_t o, typename Pol::R yr, typename Pol::R yi, typename Pol::R amp,
+__device__ __forceinline__ C2<typename Pol::R> lstep_value(size_t o, C2<typename Pol::R> b, typename Pol::R yr,
+                                                           typename Pol::R yi, typename Pol::R g, typename Pol::P rate,
+                                                           typename Pol::P amp, double dz)
+{
+    using R = typename Pol::R;
+    using P = typename Pol::P;
+    P sn, cs;
+    lstep_phasor<Pol>((P)dz * rate, &sn, &cs);
+    if constexpr (sizeof(P) > sizeof(R)) {
+        const double dg = dz * (double)g;
+        const double br = fma(dg, (double)yr, (double)b.x), bi = fma(dg, (double)yi, (double)b.y);
+        const double fx = amp * (br * cs - bi * sn), fy = amp * (br * sn + bi * cs);
+        const uint32_t h = mix_bits((uint32_t)o * 0x9e3779b1u ^ __float_as_uint(b.x) ^ (__float_as_uint(yi) << 7) ^
+                                    (__float_as_uint(yr) * 0x27d4eb2fu));
+        return C2<R>{round_unbiased(fx, h), round_unbiased(fy, (h >> 16) | (h << 16))};
+    } else {
+        const R dg = (R)dz * g;
+        const R br = b.x + dg * yr, bi = b.y + dg * yi;
+        const R a = (R)amp, c = (R)cs, s = (R)sn;
+        return C2<R>{a * (br * c - bi * s), a * (br * s + bi * c)};
+    }
+}
+
+template <class Pol>
+__device__ __forceinline__ void lstep_point(size_t o, typename Pol::R yr, typename Pol::R yi, typename Pol::P amp,
                                             const LstepArgs& la)
 {
     using R = typename Pol::R;
@@ -358,12 +423,7 @@ __device__ __forceinline__ void lstep_point(size_t o, typename Pol::R yr, typena
     C2<R>* fk = (C2<R>*)la.fk;
     const R g = ((const R*)la.gfac)[o];
     if (g < (R)0) { fk[o] = C2<R>{0, 0}; return; }
-    R sn, cs;
-    lstep_phasor<Pol>((P)la.dz * ((const P*)la.rate)[o], &sn, &cs);
-    const C2<R> b = fk[o];
-    const R dg = (R)la.dz * g;
-    const R br = b.x + dg * yr, bi = b.y + dg * yi;
-    fk[o] = C2<R>{amp * (br * cs - bi * sn), amp * (br * sn + bi * cs)};
+    fk[o] = lstep_value<Pol>(o, fk[o], yr, yi, g, ((const P*)la.rate)[o], amp, la.dz);
 }
 
 // lstep_point over NB points with all table and state loads issued before any
@@ -371,7 +431,7 @@ __device__ __forceinline__ void lstep_point(size_t o, typename Pol::R yr, typena
 // serialise one memory round trip per point)
 template <class Pol, int NB>
 __device__ __forceinline__ void lstep_batch(const size_t (&o)[NB], const typename Pol::R (&yr)[NB],
-                                            const typename Pol::R (&yi)[NB], typename Pol::R amp, const LstepArgs& la)
+                                            const typename Pol::R (&yi)[NB], typename Pol::P amp, const LstepArgs& la)
 {
     using R = typename Pol::R;
     using P = typename Pol::P;
@@ -385,25 +445,21 @@ __device__ __forceinline__ void lstep_batch(const size_t (&o)[NB], const typenam
         rt[j] = ((const P*)la.rate)[o[j]];
         b[j] = fk[o[j]];
     }
+    // a real branch, not a select: whole waves of masked columns skip the update
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-        if (g[j] < (R)0) {
-            fk[o[j]] = C2<R>{0, 0};
-            continue;
-        }
-        R sn, cs;
-        lstep_phasor<Pol>((P)la.dz * rt[j], &sn, &cs);
-        const R dg = (R)la.dz * g[j];
-        const R br = b[j].x + dg * yr[j], bi = b[j].y + dg * yi[j];
-        fk[o[j]] = C2<R>{amp * (br * cs - bi * sn), amp * (br * sn + bi * cs)};
+        C2<R> v = {0, 0};
+        if (g[j] >= (R)0) v = lstep_value<Pol>(o[j], b[j], yr[j], yi[j], g[j], rt[j], amp, la.dz);
+        fk[o[j]] = v;
     }
 }
 
+// e^{dz Im k(omega)} of a column, in the phase precision
 template <class Pol>
-__device__ __forceinline__ typename Pol::R lstep_amp(int w, const LstepArgs& la)
+__device__ __forceinline__ typename Pol::P lstep_amp(int w, const LstepArgs& la)
 {
     using P = typename Pol::P;
-    return (typename Pol::R)exp((P)la.dz * (P)la.k2[w]);
+    return (P)exp((P)la.dz * (P)la.k2[w]);
 }
 
 template <class Pol>

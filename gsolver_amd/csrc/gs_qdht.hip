@@ -320,7 +320,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) {
         const int col = n0 + wn + cb * 32 + (lane & 31);
-        const float amp = LSTEP ? lstep_amp<PolMix>(col, la) : 1.f;
+        const double amp = LSTEP ? lstep_amp<PolMix>(col, la) : 1.0;
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
