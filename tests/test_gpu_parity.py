@@ -428,3 +428,24 @@ def test_unsupported_line_length_is_rejected(tmp_path, nt):
     with pytest.raises(gsa.GsError, match="N_T must be a power of two between 256 and 4096") as ei:
         gsa.Plan(cfg, precision="mixed")
     assert ei.value.code == -1
+
+
+def test_mixed_state_keeps_sub_ulp_increments(tmp_path):
+    """100 000 fixed steps at dz = 3e-13 m (the config's first step is 1.5e-8 m
+    at tolerance 1e-3, so the nonlinear increment dz g B1 of BIGBUFFER2 is
+    ~2e-8 of its value even at the peak: below half an FP32 ulp).  A state
+    rounded to nearest in FP32 loses that increment every step, and the
+    nonlinear change of the whole run (~4e-4 at the peak) goes missing; the
+    mixed policy's FP64 update with unbiased rounding keeps it, its field
+    staying within a random walk of FP64's.  The FP32 policy (round to
+    nearest) is printed for contrast."""
+    cfg = write_cfg(tmp_path / "c.txt", **dict(HOT, N_T="256"))
+    out = {}
+    for prec in ("fp64", "mixed", "fp32"):
+        p = gsa.Plan(cfg, precision=prec)
+        p.fixed_steps(3e-13, 100000, timings=False)
+        out[prec] = p.get()
+        p.close()
+    e_mix, e_32 = rel_l2(out["mixed"], out["fp64"]), rel_l2(out["fp32"], out["fp64"])
+    print("sub-ulp increments: mixed %.2e, fp32 %.2e" % (e_mix, e_32))
+    assert e_mix < 2e-5
