@@ -361,7 +361,7 @@ __device__ __forceinline__ void lstep_phasor(typename Pol::P ph, typename Pol::P
 
 // 32 well-mixed bits from a point index and the bits of its inputs (the
 // finaliser of MurmurHash3); deterministic, so reruns are bit-identical
-__device__ __forceinline__ uint32_t mix_bits(uint32_t h)
+__host__ __device__ __forceinline__ uint32_t mix_bits(uint32_t h)
 {
     h ^= h >> 16;
     h *= 0x85ebca6bu;
@@ -378,11 +378,11 @@ __device__ __forceinline__ uint32_t mix_bits(uint32_t h)
 // mean.  A float state updated by steps whose increments fall below half an
 // ulp (round to nearest drops them every step) then drifts as a random walk
 // instead of coherently.  Exact for every x in the FP32 normal range.
-__device__ __forceinline__ float round_unbiased(double x, uint32_t u)
+__host__ __device__ __forceinline__ float round_unbiased(double x, uint32_t u)
 {
-    unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    unsigned long long b = __builtin_bit_cast(unsigned long long, x);
     b = (b + (u & 0x1fffffffu)) & ~0x1fffffffull;
-    return (float)__longlong_as_double((long long)b);
+    return (float)__builtin_bit_cast(double, b);
 }
 
 // one point of calculate_Lstep: B2 += dz B1k g; FIELD = B2 e^{i dz rate} amp; B2 = FIELD,
