@@ -216,18 +216,8 @@ def batch_check(gsa, workloads, work, nruns, z_max, seed=20261202):
     (gs_march_many); reports aggregate accepted z-steps/s of both and the
     largest difference between a run's batched and single field."""
     import numpy as np
-    rng = np.random.default_rng(seed)
     d = os.path.join(work, "batch")
-    os.makedirs(d, exist_ok=True)
-    cfgs = []
-    for i in range(nruns):
-        e = float(np.exp(rng.uniform(np.log(0.5e-6), np.log(10e-6))))
-        w0 = float(rng.uniform(40e-6, 150e-6))
-        nr = int(rng.choice([128, 256, 512]))
-        nt = int(rng.choice([512, 1024, 2048]))
-        cfgs.append(workloads.write_config(d, "c0", "run%02d" % i, N_R=nr, N_T=nt, ENERGY="%.6e" % e,
-                                           DIAMETER_FWHM="%.6e" % (w0 * np.sqrt(2 * np.log(2))),
-                                           N_Z=2, Z_MAX=z_max))
+    cfgs = workloads.batch_configs(d, nruns, z_max, seed)
     single, t_single, steps = [], 0.0, 0
     for c in cfgs:
         p = gsa.Plan(c, precision="mixed")
@@ -273,7 +263,7 @@ def gradient_check(gsa, workloads, work, z_max, ndir, energy, max_steps, objecti
     import numpy as np
     import torch
     from gsolver_amd.adjoint import PhaseMarch, record_steps
-    cfg = workloads.write_config(work, "c0", "c3", N_R=512, N_T=2048, N_Z=2, Z_MAX=z_max, ENERGY="%g" % energy)
+    cfg = workloads.write_config(work, "c3", Z_MAX=z_max, ENERGY="%g" % energy)
     rec = gsa.Plan(cfg, precision="fp64")
     t0 = time.perf_counter()
     dz_rk, dz_lin = record_steps(rec, z_end=z_max, max_steps=max_steps)

@@ -26,6 +26,10 @@ def record_steps(plan, n_steps=None, z_end=None, max_steps=None):
     runs at the previous ZSTEP, the linear step at the new one."""
     if (n_steps is None) == (z_end is None):
         raise ValueError("give exactly one of n_steps and z_end")
+    if z_end is not None:
+        znet = plan.table("znet")
+        if z_end > znet[-1]:
+            raise ValueError("z_end %.6g lies beyond the last plane ZNET[-1] = %.6g" % (z_end, znet[-1]))
     rk, lin = [], []
     cap = max_steps if max_steps is not None else float("inf")
     while ((len(rk) < n_steps) if n_steps is not None
@@ -33,6 +37,9 @@ def record_steps(plan, n_steps=None, z_end=None, max_steps=None):
         plan.step()
         rk.append(plan.scalar("LAST_DZ_NL"))
         lin.append(plan.scalar("ZSTEP"))
+        if not lin[-1] > 0:
+            raise RuntimeError("record_steps: the controller returned a zero step at z = %.6g"
+                               % plan.scalar("CURRENT_Z"))
     return np.array(rk), np.array(lin)
 
 

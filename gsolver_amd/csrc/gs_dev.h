@@ -390,10 +390,13 @@ __host__ __device__ __forceinline__ float round_unbiased(double x, uint32_t u)
 // MIXED (FP32 state, FP64 phase) updates in FP64 and stores the state with
 // unbiased rounding: at small dz the per-step B1 increment is below half an
 // FP32 ulp of B2 on most points, and round-to-nearest would drop it every step.
+// The rounding draw hashes the step's seed (its STEP_N) with the point index
+// and the point's inputs, so a point whose inputs repeat from step to step
+// (no nonlinearity, sub-ulp phase rotation) still gets a fresh draw each step.
 template <class Pol>
 __device__ __forceinline__ C2<typename Pol::R> lstep_value(size_t o, C2<typename Pol::R> b, typename Pol::R yr,
                                                            typename Pol::R yi, typename Pol::R g, typename Pol::P rate,
-                                                           typename Pol::P amp, double dz)
+                                                           typename Pol::P amp, double dz, uint32_t seed)
 {
     using R = typename Pol::R;
     using P = typename Pol::P;
@@ -403,8 +406,8 @@ __device__ __forceinline__ C2<typename Pol::R> lstep_value(size_t o, C2<typename
         const double dg = dz * (double)g;
         const double br = fma(dg, (double)yr, (double)b.x), bi = fma(dg, (double)yi, (double)b.y);
         const double fx = amp * (br * cs - bi * sn), fy = amp * (br * sn + bi * cs);
-        const uint32_t h = mix_bits((uint32_t)o * 0x9e3779b1u ^ __float_as_uint(b.x) ^ (__float_as_uint(yi) << 7) ^
-                                    (__float_as_uint(yr) * 0x27d4eb2fu));
+        const uint32_t h = mix_bits(((uint32_t)o * 0x9e3779b1u + mix_bits(seed * 0xc2b2ae3du + 0x165667b1u)) ^
+                                    __float_as_uint(b.x) ^ (__float_as_uint(yi) << 7) ^ (__float_as_uint(yr) * 0x27d4eb2fu));
         return C2<R>{round_unbiased(fx, h), round_unbiased(fy, (h >> 16) | (h << 16))};
     } else {
         const R dg = (R)dz * g;
@@ -423,7 +426,7 @@ __device__ __forceinline__ void lstep_point(size_t o, typename Pol::R yr, typena
     C2<R>* fk = (C2<R>*)la.fk;
     const R g = ((const R*)la.gfac)[o];
     if (g < (R)0) { fk[o] = C2<R>{0, 0}; return; }
-    fk[o] = lstep_value<Pol>(o, fk[o], yr, yi, g, ((const P*)la.rate)[o], amp, la.dz);
+    fk[o] = lstep_value<Pol>(o, fk[o], yr, yi, g, ((const P*)la.rate)[o], amp, la.dz, la.seed);
 }
 
 // lstep_point over NB points with all table and state loads issued before any
@@ -449,7 +452,7 @@ __device__ __forceinline__ void lstep_batch(const size_t (&o)[NB], const typenam
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
         C2<R> v = {0, 0};
-        if (g[j] >= (R)0) v = lstep_value<Pol>(o[j], b[j], yr[j], yi[j], g[j], rt[j], amp, la.dz);
+        if (g[j] >= (R)0) v = lstep_value<Pol>(o[j], b[j], yr[j], yi[j], g[j], rt[j], amp, la.dz, la.seed);
         fk[o[j]] = v;
     }
 }

@@ -82,7 +82,9 @@ struct LstepArgs {
     const void* rate;           // [rp][T], P precision of the policy
     const void* gfac;           // [rp][T], field precision
     void* fk;                   // FIELD in k-space = B2 on unmasked points (C2<R>), updated in place
-    const DevCtl* ctl;          // device-driven march: dz = ctl->dz_lin; skip unless accepted
+    const DevCtl* ctl;          // device-driven march: dz = ctl->dz_lin, seed = ctl->steps; skip unless accepted
+    uint32_t seed;              // the step's STEP_N (attempts before this linear step): salt of the
+                                // mixed policy's unbiased-rounding draws
 };
 
 // sqrt(1+x) - 1 as the 20-term Taylor sum of extmath.h:245-252, Horner form

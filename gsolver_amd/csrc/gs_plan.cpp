@@ -132,6 +132,7 @@ struct gs_plan {
     int adj_seg = 0;                       // steps per segment
     double adj_budget = -1;                // bytes of per-step checkpoints kept without recomputation
                                            // (< 0: 60 % of the device memory free at the forward)
+    long adj_seed0 = 0;                    // STEP_N at the first recorded step (rounding seeds of the replay)
     double last_dz_nl = 0;     // NL step size of the last accepted gs_step attempt
 
     void* buf(int id)
@@ -487,6 +488,7 @@ void nl_step(gs_plan* p, double dz, double* maxA2, double* maxNL2)
 void linear_half(gs_plan* p, double dz)
 {
     p->ls.dz = dz;
+    p->ls.seed = (uint32_t)p->STEP_N;
     HCK(hipEventRecord(p->ev[2], p->st));
     LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, p->prec == gs::kFP64 ? p->at64.p : p->amat.p, p->b1.p,
                         nullptr, &p->ls, p->st));
@@ -849,6 +851,8 @@ int gs_set_scalar(gs_plan* p, const char* name, double v)
     if (n == "ZSTEP") p->ZSTEP = v;
     else if (n == "CURRENT_Z") p->CURRENT_Z = v;
     else if (n == "n_Z") {
+        if (!(v >= 0 && v <= p->s.NZ - 1) || v != (double)(int)v)
+            return fail(GS_E_ARG, "n_Z must be an integer plane index in 0..N_Z-1");
         p->n_Z = (int)v;
         if (p->n_Z == 0) p->dumps_started = false;   // a march from plane 0 starts its files again
     }
@@ -997,6 +1001,7 @@ int gs_linear_step(gs_plan* p, double dz)
     return guarded([&]() {
         if (!p) return fail(GS_E_ARG, "null plan");
         p->ls.dz = dz;
+        p->ls.seed = (uint32_t)p->STEP_N;
         LCK(gs::launch_lstep(p->prec, p->rp, p->s.NT, p->b1.p, p->ls, p->st));
         sync(p);
         return 0;
@@ -1007,7 +1012,10 @@ int gs_step(gs_plan* p)
 {
     return guarded([&]() {
         if (!p) return fail(GS_E_ARG, "null plan");
-        if (p->n_Z == 0) p->n_Z = 1;
+        // the main loop's plane bookkeeping (solver_main.cpp:206-212), without the dumps
+        const auto& Z = p->s.ZNET;
+        while (p->n_Z < p->s.NZ - 1 && p->CURRENT_Z >= Z[p->n_Z]) p->n_Z++;
+        if (p->CURRENT_Z >= Z[p->s.NZ - 1]) return fail(GS_E_ARG, "gs_step: the march is at its last plane");
         propagator(p);
         p->STEP_N++;
         p->CURRENT_Z += p->ZSTEP;
@@ -1207,6 +1215,7 @@ int gs_fixed_steps(gs_plan* p, double dz, int n, double* ms4)
             for (int k = 0; k < n; ++k) {
                 LCK(gs::launch_nl_rk4(p->prec, p->s.NT, p->field.p, p->b1.p, dz, p->nl, p->linemax.as<double>(), p->st));
                 LCK(gs::launch_reduce_max(p->s.NR, p->linemax.as<double>(), p->status.as<gs::DevStatus>(), p->st));
+                p->ls.seed = (uint32_t)(p->STEP_N + k);
                 LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->b1.p, nullptr, &p->ls, p->st));
                 LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->fk.p, p->field.p, nullptr, p->st));
             }
@@ -1229,6 +1238,7 @@ int gs_fixed_steps(gs_plan* p, double dz, int n, double* ms4)
             HCK(hipEventRecord(e[1], p->st));
             LCK(gs::launch_reduce_max(p->s.NR, p->linemax.as<double>(), p->status.as<gs::DevStatus>(), p->st));
             HCK(hipEventRecord(e[2], p->st));
+            p->ls.seed = (uint32_t)(p->STEP_N + k);
             LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->b1.p, nullptr, &p->ls, p->st));
             HCK(hipEventRecord(e[3], p->st));
             LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->fk.p, p->field.p, nullptr, p->st));
@@ -1328,11 +1338,13 @@ int gs_adjoint_forward(gs_plan* p, int n, const double* dz_rk, const double* dz_
             }
             LCK(gs::launch_nl_rk4(p->prec, nt, p->field.p, p->b1.p, dz_rk[k], p->nl, p->linemax.as<double>(), p->st));
             ls.dz = dz_lin[k];
+            ls.seed = (uint32_t)(p->STEP_N + k);
             LCK(gs::launch_qdht(p->prec, rp, nt, A, p->b1.p, nullptr, &ls, p->st));
             LCK(gs::launch_qdht(p->prec, rp, nt, A, p->fk.p, p->field.p, nullptr, p->st));
         }
         check_status(p, nullptr);
         p->adj_rk.assign(dz_rk, dz_rk + n);
+        p->adj_seed0 = p->STEP_N;
         p->adj_lin.assign(dz_lin, dz_lin + n);
         for (int k = 0; k < n; ++k) p->CURRENT_Z += dz_lin[k];
         p->ZSTEP = dz_lin[n - 1];
@@ -1369,6 +1381,7 @@ int gs_adjoint_backward(gs_plan* p, const double* gfield, const double* gb2, int
                     if (k + 1 == k1) break;    // the segment's last FIELD is all the reverse sweep needs
                     LCK(gs::launch_nl_rk4(p->prec, nt, p->rfield.p, p->rb1.p, p->adj_rk[k], p->nl, p->linemax.as<double>(), p->st));
                     lr.dz = p->adj_lin[k];
+                    lr.seed = (uint32_t)(p->adj_seed0 + k);
                     LCK(gs::launch_qdht(p->prec, rp, nt, A, p->rb1.p, nullptr, &lr, p->st));
                     LCK(gs::launch_qdht(p->prec, rp, nt, A, p->rfk.p, p->rfield.p, nullptr, p->st));
                 }

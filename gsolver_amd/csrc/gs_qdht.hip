@@ -45,6 +45,7 @@ __global__ __launch_bounds__(256) void k_qdht_f32(const float* __restrict__ At, 
     if (la.ctl) {
         if (la.ctl->state != 0 || !la.ctl->accept) return;
         la.dz = la.ctl->dz_lin;
+        la.seed = (uint32_t)la.ctl->steps;
     }
     __shared__ float As[BK][BM + PA];
     __shared__ float Br[BK][BNC + PB];
@@ -139,9 +140,6 @@ __device__ __forceinline__ void split2(float v0, float v1, uint32_t* hi, uint32_
 // step ahead.  B (the field, FP32) is loaded one column per lane (16 k rows,
 // coalesced across lanes) two K steps ahead, split into bf16 hi/lo and stored
 // k-contiguous in a double-buffered LDS tile; one barrier per K step.
-#ifndef GS_GEMM_SCHED
-#define GS_GEMM_SCHED 1
-#endif
 namespace t128 {
 constexpr int BM = 128, BNC = 128, BK = 32, LD = BK + 8;
 constexpr int PLANE = BNC * LD;         // elements of one [128 columns][LD] plane
@@ -207,6 +205,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     if (la.ctl) {
         if (la.ctl->state != 0 || !la.ctl->accept) return;
         la.dz = la.ctl->dz_lin;
+        la.seed = (uint32_t)la.ctl->steps;
     }
 
     const int M = K;
@@ -277,7 +276,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     // interleave the 48 MFMAs of a step with the split VALU, the LDS writes
     // and the global loads of the other stages
     auto schedule = [&]() {
-        if constexpr (GS_GEMM_SCHED == 0) return;
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
@@ -355,6 +353,7 @@ __global__ __launch_bounds__(256) void k_qdht_f64(const double* __restrict__ At,
     if (la.ctl) {
         if (la.ctl->state != 0 || !la.ctl->accept) return;
         la.dz = la.ctl->dz_lin;
+        la.seed = (uint32_t)la.ctl->steps;
     }
     __shared__ double As[BK][BM + PA];
     __shared__ double Br[BK][BNC + PB];

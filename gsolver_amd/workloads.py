@@ -10,6 +10,9 @@
   "c1"  1024 x 4096, 1 mm / 800 fs, 3 cm, 5 uJ 40 fs, w0 = 100 um, 1 % seeded
         amplitude noise (NOISE_MAP_LEVEL / NOISE_SEED)
   "c2"  c1 with dumps at 0, 1, 2, 3 cm
+  "c3"  the c0 medium and pulse on 512 x 2048 over 1 cm (the reverse-mode
+        workload; the pulse energy is a keyword, see write_config)
+- batch_configs(): the energy / width scan of 64 seeded runs over uneven grids.
 """
 import os
 
@@ -49,6 +52,8 @@ def workload_keys(name):
         return dict(common, N_T="1024", T_MIN="-200e-15", T_MAX="200e-15", N_R="256", R_MAX="0.5e-3", N_Z="3",
                     Z_MIN="0", Z_MAX="2e-2", DURATION_FWHM="50e-15", DIAMETER_FWHM="%.6e" % (60e-6 * fw),
                     ENERGY="1e-6")
+    if name == "c3":
+        return dict(workload_keys("c0"), N_R="512", N_T="2048", N_Z="2", Z_MAX="1e-2")
     if name in ("c1", "c2"):
         k = dict(common, N_T="4096", T_MIN="-400e-15", T_MAX="400e-15", N_R="1024", R_MAX="1e-3", N_Z="4",
                  Z_MIN="0", Z_MAX="3e-2", DURATION_FWHM="40e-15", DIAMETER_FWHM="%.6e" % (100e-6 * fw),
@@ -74,3 +79,23 @@ def write_config(directory, name, out_name=None, **overrides):
         for k, v in keys.items():
             f.write("%s %s\n" % (k, v))
     return path
+
+
+def batch_configs(directory, nruns=64, z_max=1e-2, seed=20261202, **overrides):
+    """The energy / width scan: nruns runs of the c0 medium and pulse with the
+    energy log-uniform in 0.5-10 uJ, w0 uniform in 40-150 um, N_R drawn from
+    {128, 256, 512} and N_T from {512, 1024, 2048}, all from one seed, each
+    marched to z_max (planes at 0 and z_max).  Returns the config paths."""
+    rng = np.random.default_rng(seed)
+    os.makedirs(directory, exist_ok=True)
+    out = []
+    for i in range(nruns):
+        e = float(np.exp(rng.uniform(np.log(0.5e-6), np.log(10e-6))))
+        w0 = float(rng.uniform(40e-6, 150e-6))
+        nr = int(rng.choice([128, 256, 512]))
+        nt = int(rng.choice([512, 1024, 2048]))
+        keys = dict(N_R=nr, N_T=nt, ENERGY="%.6e" % e, DIAMETER_FWHM="%.6e" % (w0 * np.sqrt(2 * np.log(2))),
+                    N_Z=2, Z_MAX=z_max)
+        keys.update(overrides)
+        out.append(write_config(directory, "c0", "run%02d" % i, **keys))
+    return out

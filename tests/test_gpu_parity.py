@@ -449,3 +449,31 @@ def test_mixed_state_keeps_sub_ulp_increments(tmp_path):
     e_mix, e_32 = rel_l2(out["mixed"], out["fp64"]), rel_l2(out["fp32"], out["fp64"])
     print("sub-ulp increments: mixed %.2e, fp32 %.2e" % (e_mix, e_32))
     assert e_mix < 2e-5
+    # the test tells unbiased rounding from round to nearest: the FP32 state,
+    # rounded to nearest, misses the bound by a clear margin
+    assert e_32 > 2 * e_mix
+
+
+def test_mixed_linear_phase_drift_without_nonlinearity(tmp_path):
+    """A linear march (NONLIN_REFRINDEX 0, no ionization at 1 nJ, so B1 = 0) at
+    dz = 1e-13 m: each step rotates the phase of every point by dz (-HO - s k1),
+    ~1e-8 rad near the carrier, below half an FP32 ulp.  The inputs of a point
+    then repeat from step to step apart from the state itself, so the
+    unbiased-rounding draw must change with the step (its STEP_N seed), or a
+    point whose first draw keeps its value stays frozen.  Over 100 000 steps the
+    mixed field follows FP64's phase drift to a random walk's error, while the
+    FP32 state rounded to nearest loses most of the drift."""
+    cfg = write_cfg(tmp_path / "c.txt", **dict(HOT, N_T="256", ENERGY="1e-9", NONLIN_REFRINDEX="0"))
+    out = {}
+    for prec in ("fp64", "mixed", "fp32"):
+        p = gsa.Plan(cfg, precision=prec)
+        f0 = p.get()
+        p.fixed_steps(1e-13, 100000, timings=False)
+        out[prec] = p.get()
+        p.close()
+    drift = rel_l2(out["fp64"], f0)
+    e_mix, e_32 = rel_l2(out["mixed"], out["fp64"]), rel_l2(out["fp32"], out["fp64"])
+    print("linear phase drift %.2e: mixed error %.2e, fp32 error %.2e" % (drift, e_mix, e_32))
+    assert drift > 1e-4
+    assert e_mix < 0.05 * drift
+    assert e_32 > 0.3 * drift
