@@ -50,6 +50,7 @@ def lib():
         L.gs_put.argtypes = [vp, ip, D]
         L.gs_nl_step.argtypes = [vp, dp, D, D]
         L.gs_nl_response.argtypes = [vp, D, D, ip]
+        L.gs_nl_response_noalloc.argtypes = [vp, D, D, ip]
         L.gs_qdht.argtypes = [vp, ip]
         L.gs_linear_step.argtypes = [vp, dp]
         L.gs_step.argtypes = [vp]
@@ -181,6 +182,13 @@ class Plan:
         x = np.ascontiguousarray(np.atleast_2d(lines), dtype=np.complex128)
         out = np.empty_like(x)
         _check(lib().gs_nl_response(self.h, _d(x.view(np.float64)), _d(out.view(np.float64)), x.shape[0]))
+        return out
+
+    def nl_response_noalloc(self, lines):
+        """nl_response through the allocation-free entry point (fixed device scratch)."""
+        x = np.ascontiguousarray(np.atleast_2d(lines), dtype=np.complex128)
+        out = np.empty_like(x)
+        _check(lib().gs_nl_response_noalloc(self.h, _d(x.view(np.float64)), _d(out.view(np.float64)), x.shape[0]))
         return out
 
     def qdht(self, buf):

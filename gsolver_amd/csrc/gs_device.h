@@ -100,6 +100,17 @@ __host__ __device__ inline double sqrtho(double x)
     return r * x;
 }
 
+// Fixed device scratch of the allocation-free entry point gs_nl_response_noalloc:
+// at most kNoAllocMaxSamples line samples per call (n_lines * N_T), staged as the
+// FP64 host copy plus the field-precision input and output (16 + 2 x 16 bytes
+// per sample at most, each region 256-byte aligned).
+constexpr size_t kNoAllocMaxSamples = 262144;
+constexpr size_t kNoAllocScratchBytes = 48 * kNoAllocMaxSamples + 1024;
+// Device address of that scratch on the current device: hipGetSymbolAddress
+// runs once per device (with it current) and the address is kept per device.
+// Returns 0 or a HIP error code.
+int noalloc_scratch(void** ptr);
+
 // ---- launchers (all asynchronous on `st`) ----
 // RK4 NL step over nr lines of field -> b1; per-line maxima into linemax[2*r]
 int launch_nl_rk4(Prec p, int nt, const void* field, void* b1, double dz, const NLArgs& a, double* linemax, hipStream_t st);

@@ -5,7 +5,37 @@
 // are in gs_line.h.
 #include "gs_line.h"
 
+#include <mutex>
+
+// The allocation-free entry point's staging area: one copy per GPU, held until
+// the process ends.  Host code never passes this name to a kernel or a copy
+// (in host code it is a stand-in, not a device address); it uses the address
+// noalloc_scratch() returns for the current device.
+static __device__ __align__(256) unsigned char scratch[gs::kNoAllocScratchBytes];
+
 namespace gs {
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int noalloc_scratch(void** ptr)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return (int)hipErrorInvalidSymbol;
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return 0;
+}
 
 // FP32 field lines keep the RK4 input row A and the k1 + 2k2 + 2k3 + k4
 // accumulator in LDS (2 x 32 KiB at N_T = 4096, after LineSmem); FP64 lines

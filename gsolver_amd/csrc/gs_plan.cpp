@@ -94,6 +94,7 @@ float bf16_to_float(uint16_t h)
 struct gs_plan {
     gs::Setup s;
     gs::Prec prec = gs::kFP32;
+    int dev = 0;             // the device current at plan creation (the plan's memory and stream)
     int rp = 0;              // padded rows
     size_t relem = 0;        // bytes per complex element in the field precision
     hipStream_t st = nullptr;
@@ -233,6 +234,13 @@ void build_plan(gs_plan* p)
     const int rp = p->rp;
     const bool dbl = p->prec == gs::kFP64;
     p->relem = dbl ? 16 : 8;
+    HCK(hipGetDevice(&p->dev));
+    {
+        // address of this device's fixed scratch of gs_nl_response_noalloc, looked up once per
+        // device; a failure is reported by that entry point (which looks it up again), not here
+        void* scr = nullptr;
+        (void)gs::noalloc_scratch(&scr);
+    }
     HCK(hipStreamCreate(&p->st));
     for (auto& e : p->ev) HCK(hipEventCreate(&e));
 
@@ -970,6 +978,35 @@ int gs_nl_response(gs_plan* p, const double* in, double* out, int n_lines)
         LCK(gs::launch_nl_response(p->prec, nt, din.p, dout.p, n_lines, p->nl, p->st));
         LCK(gs::launch_convert(p->prec, d64.as<double>(), dout.p, n, 0, p->st));
         HCK(hipMemcpyAsync(out, d64.p, n * 16, hipMemcpyDeviceToHost, p->st));
+        check_status(p, nullptr);
+        return 0;
+    });
+}
+
+int gs_nl_response_noalloc(gs_plan* p, const double* in, double* out, int n_lines)
+{
+    return guarded([&]() {
+        if (!p || !in || !out || n_lines < 1) return fail(GS_E_ARG, "bad argument");
+        const int nt = p->s.NT;
+        const size_t n = (size_t)n_lines * nt;
+        if (n > gs::kNoAllocMaxSamples)
+            return fail(GS_E_ARG, "gs_nl_response_noalloc: n_lines * N_T = " + std::to_string(n) +
+                                      " exceeds the fixed scratch of " + std::to_string(gs::kNoAllocMaxSamples) +
+                                      " samples");
+        HCK(hipSetDevice(p->dev));
+        void* base = nullptr;
+        LCK(gs::noalloc_scratch(&base));
+        // [FP64 host copy: 16 n][input, field precision][output, field precision], 256-byte aligned
+        auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+        char* sc = static_cast<char*>(base);
+        double* d64 = reinterpret_cast<double*>(sc);
+        void* din = sc + up(n * 16);
+        void* dout = sc + up(n * 16) + up(n * p->relem);
+        HCK(hipMemcpyAsync(d64, in, n * 16, hipMemcpyHostToDevice, p->st));
+        LCK(gs::launch_convert(p->prec, d64, din, n, 1, p->st));
+        LCK(gs::launch_nl_response(p->prec, nt, din, dout, n_lines, p->nl, p->st));
+        LCK(gs::launch_convert(p->prec, d64, dout, n, 0, p->st));
+        HCK(hipMemcpyAsync(out, d64, n * 16, hipMemcpyDeviceToHost, p->st));
         check_status(p, nullptr);
         return 0;
     });

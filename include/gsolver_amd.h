@@ -105,6 +105,16 @@ int gs_nl_step(gs_plan* p, double dz, double* maxA2, double* maxNL2);
 /* calculate_NLresponse (propagator.cu:629-783) for n_lines lines given in
  * host memory (n_lines * N_T complex) -> out (same shape). */
 int gs_nl_response(gs_plan* p, const double* in, double* out, int n_lines);
+/* gs_nl_response without any device allocation: the same kernels, with the
+ * FP64 staging copy and the field-precision input and output in a fixed
+ * device scratch array of the library (12 MiB per GPU, held until the process
+ * ends) instead of buffers allocated per call.  No counterpart in the
+ * reference: for callers that cannot allocate device memory per call (tight
+ * serving loops, harnesses that forbid hipMalloc).  n_lines * N_T must not
+ * exceed 262144 samples (GS_E_ARG otherwise).  All calls on one GPU share the
+ * scratch: calls that overlap on the same GPU (plans on other threads or
+ * streams) race and must be serialised by the caller. */
+int gs_nl_response_noalloc(gs_plan* p, const double* in, double* out, int n_lines);
 /* qdht_plan::run_many (fhatha.cpp:218-230) via ht_run (solver.h:337-346),
  * in place on a device buffer. */
 int gs_qdht(gs_plan* p, int buffer);
