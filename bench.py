@@ -30,13 +30,17 @@ HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"fp32": 157.3, "mixed": 2500.0, "fp64": 78.6}
 
 
-def run_precision(gsa, cfg, prec, steps, warmup, dz):
+def run_precision(gsa, cfg, prec, steps, warmup, dz, skip=True):
     """warmup untimed steps, then exactly `steps` fixed-dz steps enqueued back to
     back without timing events; the library synchronises the device before it
     returns, so the wall clock brackets complete device work.  Per-kernel device
     times come from a further, event-instrumented run of min(steps, 20) steps
-    outside the timed region (every precision runs the same number of steps)."""
+    outside the timed region (every precision runs the same number of steps).
+    skip: the march's QDHTs compute only the omega columns the linear step
+    keeps (QDHT_SKIP_COLUMNS); False transforms all N_t columns."""
     p = gsa.Plan(cfg, precision=prec)
+    p.set_scalar("QDHT_SKIP_COLUMNS", 1 if skip else 0)
+    live_cols = 32 * p.scalar("QDHT_LIVE_BLOCKS") if skip else p.T
     print("[bench] %s: line kernel %d samples per thread (autotuned)" % (prec, p.scalar("NL_SAMPLES_PER_THREAD")),
           file=sys.stderr, flush=True)
     if warmup > 0:
@@ -49,7 +53,7 @@ def run_precision(gsa, cfg, prec, steps, warmup, dz):
     R, T = p.R, p.T
     p.close()
     return {"wall_s": wall, "steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
-            "kernel_ms": kernel_ms, "field": field, "R": R, "T": T}
+            "kernel_ms": kernel_ms, "field": field, "R": R, "T": T, "qdht_columns": int(live_cols)}
 
 
 def cpu_baseline(cfg, dz, nsteps, threads):
@@ -169,6 +173,26 @@ def accuracy_check(gsa, workloads, work, z_max, planes, precisions=("fp64", "mix
 WORKLOAD = "N_r=1024 x N_t=4096, R_MAX 1 mm, 800 fs window, fused silica, 5 uJ 40 fs w0=100 um"
 WORKLOAD_C0 = "N_r=256 x N_t=1024, R_MAX 0.5 mm, 400 fs window, fused silica, 1 uJ 50 fs w0=60 um"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_1024x4096.json")
+KERNEL_SOURCES = ("gs_dev.h", "gs_device.h", "gs_line.h", "gs_line.hip", "gs_qdht.hip")
+
+
+def kernel_sources_sha():
+    """sha256 (16 hex digits) of the march kernels' sources, recorded in the
+    traffic profile so a stale profile is flagged"""
+    import hashlib
+    h = hashlib.sha256()
+    for n in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "gsolver_amd", "csrc", n), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def traffic_profile_current():
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f).get("kernel_sources_sha16") == kernel_sources_sha()
+    except (OSError, ValueError):
+        return False
 
 
 def measured_traffic(kernel):
@@ -213,8 +237,10 @@ def batch_check(gsa, workloads, work, nruns, z_max, seed=20261202):
     and N_T in {512, 1024, 2048} (seeded), the 0.5 mm / 400 fs fused-silica
     set-up, each marched to z_max in mixed precision with the reference
     controller.  Timed once run by run and once all together
-    (gs_march_many); reports aggregate accepted z-steps/s of both and the
-    largest difference between a run's batched and single field."""
+    (gs_march_many: each phase of a step attempt is one launch over every
+    run, replayed from one graph); reports aggregate accepted z-steps/s of
+    both and the largest difference between a run's batched and single
+    field."""
     import numpy as np
     d = os.path.join(work, "batch")
     cfgs = workloads.batch_configs(d, nruns, z_max, seed)
@@ -239,6 +265,58 @@ def batch_check(gsa, workloads, work, nruns, z_max, seed=20261202):
            "speedup": t_single / t_batch, "max_abs_field_diff_batched_vs_single": diff}
     print("[bench] batch: %d runs, %d steps, batched %.0f steps/s, one by one %.0f steps/s, max diff %g"
           % (nruns, steps, out["batched_steps_per_s"], out["single_steps_per_s"], diff), file=sys.stderr, flush=True)
+    return out
+
+
+def plane_march(gsa, workloads, work, prec, z_plane, state_dir, resume, time_limit, chunk=20000):
+    """The production workload (c1 set-up) marched by the reference controller
+    to the plane z_plane in one precision, in gs_march calls of `chunk` steps,
+    stopping early once `time_limit` seconds have passed.  The state (FIELD and
+    FIELD_k on the 32-column blocks the march transforms -- every other column
+    of both is zero -- CURRENT_Z, ZSTEP and the step counts) is written to
+    state_dir/<prec>_state.npz, and `resume` continues from such a file, so a
+    march longer than one session runs in pieces and ends where an
+    uninterrupted one would (FP64 exactly).  At the plane, the per-line
+    fluence TSTEP sum_t |E|^2, peak intensity and peak plasma density
+    (gs_line_diagnostics) are saved with the field."""
+    import numpy as np
+    cfg = workloads.write_config(os.path.join(work, "plane"), "c1", N_Z=2, Z_MAX=z_plane)
+    p = gsa.Plan(cfg, precision=prec)
+    om = p.table("omega")
+    live = (om >= p.scalar("OMEGA_MIN")) & (om <= p.scalar("OMEGA_MAX"))
+    cols = np.nonzero(np.repeat(np.add.reduceat(live, np.arange(0, p.T, 32)) > 0, 32))[0]
+    path = os.path.join(state_dir, "%s_state.npz" % prec)
+    steps0 = rej0 = 0
+    if resume:
+        st = np.load(path)
+        for buf, key in ((gsa.FIELD, "field"), (gsa.FK, "fk")):
+            a = np.zeros((p.R, p.T), complex)
+            a[:, st["cols"]] = st[key]
+            p.put(a, buf)
+        p.set_scalar("CURRENT_Z", float(st["z"]))
+        p.set_scalar("ZSTEP", float(st["dz"]))
+        p.set_scalar("n_Z", 1)
+        steps0, rej0 = int(st["steps"]), int(st["rejected"])
+    t0 = time.perf_counter()
+    done = False
+    while not done and time.perf_counter() - t0 < time_limit:
+        done = p.march(os.path.join(work, "plane"), chunk) == 0
+        print("[bench] plane march %s: %d steps, z = %.6g m, dz = %.3g m, %.1f s"
+              % (prec, steps0 + p.scalar("STEP_N"), p.scalar("CURRENT_Z"), p.scalar("ZSTEP"),
+                 time.perf_counter() - t0), file=sys.stderr, flush=True)
+    os.makedirs(state_dir, exist_ok=True)
+    steps, rej = steps0 + int(p.scalar("STEP_N")), rej0 + int(p.scalar("n_reject"))
+    out = {"precision": prec, "z": p.scalar("CURRENT_Z"), "steps": steps, "rejected": rej,
+           "wall_s": time.perf_counter() - t0, "reached": bool(done),
+           "scan_fallbacks": p.scalar("SCAN_FALLBACKS")}
+    f = p.get(gsa.FIELD)
+    np.savez(path, cols=cols, field=f[:, cols], fk=p.get(gsa.FK)[:, cols], z=p.scalar("CURRENT_Z"),
+             dz=p.scalar("ZSTEP"), steps=steps, rejected=rej)
+    if done:
+        flux, maxI, rho = p.line_diagnostics()
+        np.savez(os.path.join(state_dir, "%s_plane.npz" % prec), cols=cols, field=f[:, cols], flux=flux,
+                 maxI=maxI, rhomax=rho, z=p.scalar("CURRENT_Z"), steps=steps)
+    p.close()
     return out
 
 
@@ -345,9 +423,10 @@ def gradient_check(gsa, workloads, work, z_max, ndir, energy, max_steps, objecti
     return out
 
 
-def roofline(kind, ms, R, T, prec, kernel=None):
+def roofline(kind, ms, R, T, prec, kernel=None, cols=None):
     if kind == "qdht":
-        flops = 4.0 * R * R * T                 # one transform: real R x R times 2T real columns
+        # one transform: real R x R times 2 x (computed complex columns) real columns
+        flops = 4.0 * R * R * (cols or T)
         ach = flops / (ms * 1e-3) / 1e12
         peak = PEAK_TFLOPS[prec]
         return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
@@ -371,7 +450,9 @@ def main():
                     help="fixed-dz steps of the FP64 oracle timed for cpu_baseline (about 3 s each on 16 threads)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the adaptive-march accuracy check")
     ap.add_argument("--batch-runs", type=int, default=64, help="runs of the energy/width scan (0: skip)")
-    ap.add_argument("--batch-zmax", type=float, default=5e-6, help="distance of each scan run in m")
+    ap.add_argument("--batch-zmax", type=float, default=1e-4,
+                    help="distance of each scan run in m (the scan's 1 cm needs millions of steps per run "
+                         "once the pulses collapse; the default is a stated cap)")
     ap.add_argument("--accuracy-zmax", type=float, default=1e-3,
                     help="distance of the accuracy-check march in m")
     ap.add_argument("--accuracy-planes", type=int, default=2, help="dump planes of the accuracy check after z = 0")
@@ -399,6 +480,15 @@ def main():
     ap.add_argument("--components", action="store_true",
                     help="only run the stand-alone line FFT, linear step and single-RHS kernels at 1024 x 4096 "
                          "(mixed and FP32; for rocprofv3 kernel traces)")
+    ap.add_argument("--plane-march", default=None, metavar="PRECISION",
+                    help="only march the production workload to --plane-z in this precision, saving its state "
+                         "and, at the plane, its field and line diagnostics under --state-dir")
+    ap.add_argument("--plane-z", type=float, default=1e-2)
+    ap.add_argument("--state-dir", default="gpurun_out/plane")
+    ap.add_argument("--resume", action="store_true", help="continue the --plane-march from its saved state")
+    ap.add_argument("--time-limit", type=float, default=900.0, help="seconds of --plane-march before it saves")
+    ap.add_argument("--fft-modes", action="store_true",
+                    help="only compare the mixed march with its line FFTs on VALU or on MFMA (GS_MIX_FFT modes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -424,6 +514,11 @@ def main():
 
     work = tempfile.mkdtemp(prefix="gs_bench_")
     cfg = workloads.write_config(work, "c1")
+    if args.plane_march:
+        r = plane_march(gsa, workloads, work, args.plane_march, args.plane_z, args.state_dir, args.resume,
+                        args.time_limit)
+        print(json.dumps({"plane_march": r}))
+        return
     if args.gradient_only:
         grad = gradient_check(gsa, workloads, work, args.gradient_zmax, args.gradient_dirs, args.gradient_energy,
                               args.gradient_max_steps, args.gradient_objective)
@@ -464,12 +559,31 @@ def main():
     probe = gsa.Plan(cfg, precision="fp32")
     dz = probe.scalar("ZSTEP")
     probe.close()
+    if args.fft_modes:
+        ref = run_precision(gsa, cfg, "fp64", args.steps, args.warmup, dz)["field"]
+        out = {}
+        for mode in ("valu", "kerr", "fwd", "kerr_inv", "all"):
+            os.environ["GS_MIX_FFT"] = mode
+            r = run_precision(gsa, cfg, "mixed", args.steps, args.warmup, dz)
+            out[mode] = {"steps_per_s": r["steps_per_s"], "kernel_ms": r["kernel_ms"],
+                         "field_rel_l2_vs_fp64": float(np.linalg.norm(r["field"] - ref) / np.linalg.norm(ref))}
+            print("[bench] GS_MIX_FFT=%s: %s" % (mode, out[mode]), file=sys.stderr, flush=True)
+        os.environ.pop("GS_MIX_FFT")
+        print(json.dumps({"fft_modes": out}))
+        return
 
     res = {}
     for prec in ("mixed", "fp32") + (() if args.no_fp64 else ("fp64",)):
         res[prec] = run_precision(gsa, cfg, prec, args.steps, args.warmup, dz)
         print("[bench] %s: %.1f z-steps/s, kernels %s" % (prec, res[prec]["steps_per_s"],
               {k: round(v, 4) for k, v in res[prec]["kernel_ms"].items()}), file=sys.stderr, flush=True)
+    # the same march with both QDHTs over all N_t columns (no column skipping)
+    full = {}
+    for prec in ("mixed", "fp32"):
+        full[prec] = run_precision(gsa, cfg, prec, args.steps, args.warmup, dz, skip=False)
+        print("[bench] %s, all columns: %.1f z-steps/s, kernels %s" % (prec, full[prec]["steps_per_s"],
+              {k: round(v, 4) for k, v in full[prec]["kernel_ms"].items()}), file=sys.stderr, flush=True)
+        full[prec]["same_field_as_skip"] = bool(np.array_equal(full[prec]["field"], res[prec]["field"]))
 
     err = {}
     if "fp64" in res:
@@ -480,9 +594,10 @@ def main():
     m = res["mixed"]
     km = m["kernel_ms"]
     R, T = m["R"], m["T"]
+    cols = m["qdht_columns"]
     qdht_ms = 0.5 * (km["qdht_lstep"] + km["qdht_back"])
     dominant = "nl" if km["nl_rk4"] >= km["qdht_lstep"] + km["qdht_back"] else "qdht"
-    roof = (roofline("qdht", qdht_ms, R, T, "mixed", "k_qdht_bf16x3_t128<false>") if dominant == "qdht"
+    roof = (roofline("qdht", qdht_ms, R, T, "mixed", "k_qdht_bf16x3_t32<false>", cols) if dominant == "qdht"
             else roofline("nl", km["nl_rk4"], R, T, "mixed", "k_nl_rk4<PolMix>"))
 
     acc = None
@@ -511,22 +626,35 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "mixed: bf16x3-mfma/f32-acc QDHT, f32 FFT, f64 plasma scan",
+        "dtype": "mixed: bf16x3-mfma/f32-acc QDHT and forward line FFTs, f32 inverse FFT, f64 plasma scan, "
+                 "linear-step update and norms",
         "data": "synthetic (config-built analytic pulse, seeded 1% amplitude noise)",
         "config": {"workload": WORKLOAD,
                    "grid": [R, T], "dz_m": dz, "mode": "fixed-dz", "precision": "mixed"},
         "fp32_value": res["fp32"]["steps_per_s"],
         "mixed_over_fp32": m["steps_per_s"] / res["fp32"]["steps_per_s"],
+        "qdht_columns": {"computed": cols, "of": T,
+                         "rule": "32-column blocks holding an omega inside [OMEGA_MIN, OMEGA_MAX]; the linear "
+                                 "step zeroes the others (propagator.cu:601-603)"},
+        "all_columns": {"mixed": full["mixed"]["steps_per_s"], "fp32": full["fp32"]["steps_per_s"],
+                        "mixed_over_fp32": full["mixed"]["steps_per_s"] / full["fp32"]["steps_per_s"],
+                        "fields_identical_to_skipping": {k: full[k]["same_field_as_skip"] for k in full},
+                        "kernel_ms_per_step": {k: full[k]["kernel_ms"] for k in full}},
+        "fp32_long_march_note": "over the 1.06M-step 256x1024 march to 2 cm the FP32 policy (round-to-nearest "
+                                "state) ends 3.9e-2 from FP64, mixed 4.0e-5 (profiles/accuracy_256x1024_full_march.md)",
         "field_rel_l2_vs_fp64": err or None,
         "accuracy_march_vs_fp64": acc,
         "energy_width_scan": batch,
         "phase_gradient": grad,
         "kernel_ms_per_step": {p: res[p]["kernel_ms"] for p in res},
         "roofline": roof,
-        "roofline_qdht": roofline("qdht", km["qdht_back"], R, T, "mixed", "k_qdht_bf16x3_t128<false>"),
-        "roofline_qdht_lstep": roofline("qdht", km["qdht_lstep"], R, T, "mixed", "k_qdht_bf16x3_t128<true>"),
+        "roofline_qdht": roofline("qdht", km["qdht_back"], R, T, "mixed", "k_qdht_bf16x3_t32<false>", cols),
+        "roofline_qdht_lstep": roofline("qdht", km["qdht_lstep"], R, T, "mixed", "k_qdht_bf16x3_t32<true>", cols),
         "roofline_qdht_fp32": roofline("qdht", res["fp32"]["kernel_ms"]["qdht_back"], R, T, "fp32",
-                                       "k_qdht_f32<false>"),
+                                       "k_qdht_f32<false, 32>", res["fp32"]["qdht_columns"]),
+        "roofline_qdht_all_columns": roofline("qdht", full["mixed"]["kernel_ms"]["qdht_back"], R, T, "mixed",
+                                              "k_qdht_bf16x3_t128<false>"),
+        "traffic_profile_current": traffic_profile_current(),
         "roofline_nl": roofline("nl", km["nl_rk4"], R, T, "mixed", "k_nl_rk4<PolMix>"),
         "roofline_nl_valu": valu_roofline("k_nl_rk4<PolMix>", km["nl_rk4"]),
         "cpu_baseline": cpu,

@@ -434,7 +434,7 @@ __global__ __launch_bounds__((Shape<Pol, T, E>::NT)) void k_nl_rk4_vjp(const C2<
     }
 #pragma unroll
     for (int r = 0; r < E; ++r) gfield[row + tid + r * NT] = gA[r];
-    if (flags) atomicOr(&a.status->flags, flags);
+    report_flags(a.status, flags);
 }
 
 // transpose of the linear step at one point: with gfk the cotangent of FIELD_k

@@ -24,9 +24,30 @@ template <class R> __device__ __forceinline__ C2<R> cscale(C2<R> a, R s) { retur
 // multiply by sign*i
 template <int SIGN, class R> __device__ __forceinline__ C2<R> cmuli(C2<R> a) { return SIGN > 0 ? C2<R>{-a.y, a.x} : C2<R>{a.y, -a.x}; }
 
-struct PolF64 { using R = double; using S = double; using W = double; using P = double; };
-struct PolF32 { using R = float;  using S = float;  using W = float;  using P = float;  };
-struct PolMix { using R = float;  using S = double; using W = float;  using P = double; };
+// MFMA operand / result vectors
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+
+// split2: two floats -> packed bf16 hi pair and lo pair (round to nearest
+// even, v_cvt_pk_bf16_f32), hi + lo = v to about 2^-17 relative
+__device__ __forceinline__ void split2(float v0, float v1, uint32_t* hi, uint32_t* lo)
+{
+    const f32x2_t v = {v0, v1};
+    const bf16x2_t h = __builtin_convertvector(v, bf16x2_t);
+    const bf16x2_t l = __builtin_convertvector(v - __builtin_convertvector(h, f32x2_t), bf16x2_t);
+    *hi = __builtin_bit_cast(uint32_t, h);
+    *lo = __builtin_bit_cast(uint32_t, l);
+}
+
+struct PolF64 { using R = double; using S = double; using W = double; using P = double; static constexpr bool mfma_fft = false; };
+struct PolF32 { using R = float;  using S = float;  using W = float;  using P = float;  static constexpr bool mfma_fft = false; };
+struct PolMix { using R = float;  using S = double; using W = float;  using P = double; static constexpr bool mfma_fft = false; };
+// the mixed policy with its line FFTs on the matrix cores (split-BF16 radix-16
+// passes, NLArgs::mfma_fwd / mfma_inv); a separate instantiation, so the MFMA
+// code's registers do not weigh on the default (VALU) line kernel
+struct PolMixMfma : PolMix { static constexpr bool mfma_fft = true; };
 
 
 #define HIPCHK(x)                                  \
@@ -270,6 +291,16 @@ __device__ void block_max2(double& a, double& b, double* lds)
     a = lane_bcast(x, 63);
     b = lane_bcast(y, 63);
     __syncthreads();
+}
+
+// a thread's status bits into the plan's DevStatus; a line whose plasma scan
+// took the chunk-sequential fallback (only thread 0 reports that bit) also
+// counts once in fallbacks
+__device__ __forceinline__ void report_flags(DevStatus* st, int flags)
+{
+    if (!flags) return;
+    atomicOr(&st->flags, flags);
+    if (flags & kFlagScanSlow) atomicAdd(&st->fallbacks, 1);
 }
 
 // ------------------------------------------------------------------ plasma

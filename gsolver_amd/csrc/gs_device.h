@@ -43,7 +43,21 @@ struct DevCtl {
     int accept;         // this slot: 1 accepted, 0 rejected
     int state;          // 0 running, 1 reached z_plane, 2 error
     int err;            // 1 step collapse, 2 NaN intensity, 3 blast
-    int pad;
+    int dead_dirty;     // FIELD still holds values in the columns the linear step zeroes:
+                        // the next accepted QDHT back clears them (and this flag)
+};
+
+// Column selection of a QDHT GEMM that skips the omega columns the linear step
+// zeroes (omega outside [OMEGA_MIN, OMEGA_MAX], propagator.cu:601-603): only the
+// 32-column blocks listed in ctile are computed.  The QDHT back can also
+// clear FIELD on the other blocks (ztile), once: when zero_dead is set, or,
+// in the device-driven march, when DevCtl::dead_dirty is.
+struct QCols {
+    const int* ctile;   // first column of each computed 32-column block
+    int nct;
+    const int* ztile;   // first column of each skipped 32-column block
+    int nzt;
+    int zero_dead;      // host-driven launches: clear the skipped blocks of the output
 };
 
 // Scalars and table pointers the NL kernels read.  Table pointers refer to
@@ -63,6 +77,12 @@ struct NLArgs {
     const void* pf;             // PLASMA_FUNC (C2<R>)
     const void* kerr;           // KERR_PROFILE (R)
     const void* tw;             // e^{-2 pi i k/T}, k < T (C2<R>)
+    const void* dft16;          // mixed: the radix-16 DFTs (sign -1, +1) as split-BF16 MFMA A
+                                // fragments [sign][k slice][hi, lo][lane][8] (bf16); null otherwise
+    const void* mfma_tw;        // mixed: lane-order twiddles of the MFMA passes after the first,
+                                // [pass][group j][r] = e^{-2 pi i r (j mod Ns) / (16 Ns)} (C2<float>)
+    int mfma_fwd;               // mixed: forward line FFTs on MFMA (0 none, 1 the Kerr/Raman term, 2 all)
+    int mfma_inv;               // mixed: inverse line FFTs of non-ionizing lines on MFMA
     DevStatus* status;
     int nl_e;                   // samples per thread of the RK4 kernel (0: default)
     const DevCtl* ctl;          // device-driven march: dz from ctl->dz, skip when ctl->state != 0
@@ -111,6 +131,29 @@ constexpr size_t kNoAllocScratchBytes = 48 * kNoAllocMaxSamples + 1024;
 // Returns 0 or a HIP error code.
 int noalloc_scratch(void** ptr);
 
+// ---- batched march (gs_march_many): per-run descriptors in device memory
+struct NLRun {                  // the NL RK4 step of one run
+    const void* field;
+    void* b1;
+    double* linemax;
+    NLArgs a;                   // a.ctl: the run's controller
+};
+struct GemmRun {                // one QDHT of one run's step (X -> Y, the linear step fused when la.fk)
+    const void* A;
+    const void* X;
+    void* Y;
+    int K, NC;                  // padded rows, columns
+    int b0, nb;                 // the run's first block in the launch (a multiple of 8), its block count
+    LstepArgs la;               // la.ctl: the run's controller
+    QCols qc;
+};
+struct CtlRun {
+    int nr;
+    const double* linemax;
+    DevStatus* status;
+    DevCtl* ctl;
+};
+
 // ---- launchers (all asynchronous on `st`) ----
 // RK4 NL step over nr lines of field -> b1; per-line maxima into linemax[2*r]
 int launch_nl_rk4(Prec p, int nt, const void* field, void* b1, double dz, const NLArgs& a, double* linemax, hipStream_t st);
@@ -127,10 +170,21 @@ int launch_line_diag(Prec p, int nt, const void* field, int nlines, const double
 // amat: FP64/FP32 transposed matrix, or for MIXED the bf16 hi/lo split in
 // MFMA fragment order (see the plan set-up).
 int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* in, void* out,
-                const LstepArgs* lstep, hipStream_t st, const DevCtl* gate = nullptr);
+                const LstepArgs* lstep, hipStream_t st, const DevCtl* gate = nullptr, const QCols* cols = nullptr);
 // BIGBUFFER2 <-> FIELD_k: out = masked ? b2 : fk (the full B2), or fk = masked ? 0 : b2
 int launch_b2_merge(Prec p, const void* fk, const void* b2, void* out, const void* gfac, size_t n_complex, hipStream_t st);
 int launch_b2_split(Prec p, const void* b2, void* fk, const void* gfac, size_t n_complex, hipStream_t st);
+// batched phases of gs_march_many: the NL RK4 step of items[b] = (run, line) for runs
+// of one precision, line length and line shape (nl_e); one QDHT of every run of a
+// precision (brun[b]: the run of block b); the controllers (one block per run) and
+// the bookkeeping after the linear half of n runs
+int launch_nl_rk4_batch(Prec p, int nt, int nl_e, bool mfma_fft, const NLRun* runs, const int2* items, int nitems,
+                        hipStream_t st);
+int launch_qdht_batch(Prec p, bool lstep, const GemmRun* runs, const int* brun, int nblocks, hipStream_t st);
+int launch_controller_batch(const CtlRun* runs, int n, hipStream_t st);
+int launch_finish_batch(DevCtl* ctls, int n, hipStream_t st);
+// blocks of one run's QDHT in a batched launch (the single-run grid of launch_qdht)
+int qdht_blocks(Prec p, int rp, int nt, const QCols* cols, bool back, bool gated);
 // element-wise linear step from a k-space increment (calculate_Lstep)
 int launch_lstep(Prec p, int rp, int nt, const void* b1k, const LstepArgs& la, hipStream_t st);
 // max of the per-line maxima -> status

@@ -24,6 +24,8 @@
 
 #include "gs_dev.h"
 
+#include <type_traits>
+
 #ifndef GS_NL_E32
 #define GS_NL_E32 8    // samples per thread of the FP32/mixed line kernels for N_T >= 1024
 #endif
@@ -177,6 +179,198 @@ __device__ __forceinline__ void fft_inv(const C2<R> (&y)[E], C2<R> (&x)[E], C2<R
     if constexpr (Chain::out == 0) __syncthreads();   // as in fft_fwd_acc
 }
 
+// ------------------------------------------------------------------ MFMA FFT
+// Mixed policy: the line FFTs run their radix-16 passes on the matrix cores.
+// One pass maps the 16 inputs of each butterfly group j (src[j + r T/16],
+// twiddled by w^{r k}, k = j mod Ns) to its 16 outputs Y_q = sum_r x_r
+// e^{SIGN 2 pi i q r/16} as one real GEMM: [Re Y; Im Y] (32 rows) = [[C, -S],
+// [S, C]] (32 x 32, the k columns holding Re x then Im x) times 32 groups,
+// v_mfma_f32_32x32x16_bf16 with both operands split into bf16 hi + lo and the
+// three products lo*hi, hi*lo, hi*hi accumulated in FP32 (the operator
+// fragments come from the plan, dft16).  Each wave owns 32 groups: lane l
+// holds group 32 w + l%32 and inputs r = 8 (l/32) .. +7 (the B operand
+// layout); its results are Y_q for q in {4h..4h+3, 8+4h..8+4h+3}, h = l/32.
+// A transform is good to ~5e-6 relative (FP32 VALU: ~1e-7).  The forward
+// transforms of the RHS terms (loss, rho E, Kerr/Raman product) enter N
+// linearly.  The inverse transform's E(t) also feeds W(I) ~ I^6, so it runs
+// on the matrix cores only for lines whose intensity stays below the
+// ionization threshold; other lines transform again in FP32.
+// twp: this pass's twiddles w^{r k} in lane order, [group j][r] (null at Ns = 1)
+template <int T, int E, int SIGN>
+__device__ __forceinline__ void mfma_pass16(const C2<float>* __restrict__ src, C2<float>* __restrict__ dst,
+                                            const C2<float>* __restrict__ twp, int Ns, const v8bf (&fa)[2][2])
+{
+    constexpr int NG = T / 16, NW = T / E / 64;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, h = lane >> 5;
+    for (int g0 = wv * 32; g0 < NG; g0 += NW * 32) {     // wave-uniform: MFMA needs every lane
+        const int j = g0 + (lane & 31);
+        const int k = j & (Ns - 1);
+        float4 tq[4];
+        if (Ns > 1) {
+            const float4* tp = reinterpret_cast<const float4*>(twp + (size_t)j * 16 + 8 * h);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) tq[i] = tp[i];
+        }
+        float re[8], im[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            C2<float> v = src[pidx(j + NG * (8 * h + i), E)];
+            if (Ns > 1) {
+                const float4 t = tq[i >> 1];
+                C2<float> w = (i & 1) ? C2<float>{t.z, t.w} : C2<float>{t.x, t.y};
+                if (SIGN > 0) w.y = -w.y;
+                v = cmul(v, w);
+            }
+            re[i] = v.x;
+            im[i] = v.y;
+        }
+        uint32_t rh[4], rl[4], ih[4], il[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            split2(re[2 * i], re[2 * i + 1], &rh[i], &rl[i]);
+            split2(im[2 * i], im[2 * i + 1], &ih[i], &il[i]);
+        }
+        typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+        const v8bf brh = __builtin_bit_cast(v8bf, u32x4_t{rh[0], rh[1], rh[2], rh[3]});
+        const v8bf brl = __builtin_bit_cast(v8bf, u32x4_t{rl[0], rl[1], rl[2], rl[3]});
+        const v8bf bih = __builtin_bit_cast(v8bf, u32x4_t{ih[0], ih[1], ih[2], ih[3]});
+        const v8bf bil = __builtin_bit_cast(v8bf, u32x4_t{il[0], il[1], il[2], il[3]});
+        // two independent chains (the Re x and Im x halves of K), summed at the end
+        v16f a0, a1;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) a0[e] = a1[e] = 0.f;
+        a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][1], brh, a0, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][1], bih, a1, 0, 0, 0);
+        a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][0], brl, a0, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][0], bil, a1, 0, 0, 0);
+        a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][0], brh, a0, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][0], bih, a1, 0, 0, 0);
+        const int d = (j - k) * 16 + k;
+        // result row 8 (e/4) + 4 h + e%4 of the 32: Re Y_q for rows < 16, Im Y_q at row + 16 (e + 8)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int q = (e < 4 ? 0 : 8) + 4 * h + (e & 3);
+            dst[pidx(d + q * Ns, E)] = C2<float>{a0[e] + a1[e], a0[e + 8] + a1[e + 8]};
+        }
+    }
+}
+
+// The passes over M more points from buffer P: radix 16 on MFMA while 16
+// divides M, then one VALU pass of the remaining radix (2, 4 or 8).
+template <int T, int NT, int E, int M, int SIGN, int P>
+struct MfmaChain {
+    static constexpr bool MF = M >= 16 && M % 16 == 0;
+    static constexpr int RAD = MF ? 16 : M;
+    using Next = MfmaChain<T, NT, E, M / RAD, SIGN, 1 - P>;
+    static constexpr int out = Next::out;
+    // twp: the lane-order twiddle tables of the MFMA passes after the first
+    // ([pass][group][r], T entries each); tw: the plain table for a VALU pass
+    static __device__ __forceinline__ void run(C2<float>* b0, C2<float>* b1, const C2<float>* tw,
+                                               const C2<float>* twp, const v8bf (&fa)[2][2], int Ns)
+    {
+        if constexpr (MF) {
+            mfma_pass16<T, E, SIGN>(P ? b1 : b0, P ? b0 : b1, Ns > 1 ? twp : nullptr, Ns, fa);
+            __syncthreads();
+            Next::run(b0, b1, tw, Ns > 1 ? twp + T : twp, fa, Ns * RAD);
+        } else {
+            spass<float, T, NT, E, RAD, SIGN>(P ? b1 : b0, P ? b0 : b1, tw, Ns);
+            __syncthreads();
+            Next::run(b0, b1, tw, twp, fa, Ns * RAD);
+        }
+    }
+};
+template <int T, int NT, int E, int SIGN, int P>
+struct MfmaChain<T, NT, E, 1, SIGN, P> {
+    static constexpr int out = P;
+    static __device__ __forceinline__ void run(C2<float>*, C2<float>*, const C2<float>*, const C2<float>*,
+                                               const v8bf (&)[2][2], int) {}
+};
+
+// the operator fragments of sign SIGN (dft16 holds [sign: -1, +1][k slice][hi, lo][lane][8])
+template <int SIGN>
+__device__ __forceinline__ void load_dft16(const uint16_t* dft16, v8bf (&fa)[2][2])
+{
+    const int lane = threadIdx.x & 63;
+    const uint16_t* f = dft16 + (SIGN > 0 ? 2 * 2 * 64 * 8 : 0);
+#pragma unroll
+    for (int sl = 0; sl < 2; ++sl)
+#pragma unroll
+        for (int hl = 0; hl < 2; ++hl) fa[sl][hl] = *(const v8bf*)(f + ((sl * 2 + hl) * 64 + lane) * 8);
+}
+
+// fft_inv with the passes of MfmaChain (FP32 data, T >= 512): strided-layout
+// spectrum in, chunk-layout time signal out, unnormalised
+template <int T, int NT, int E>
+__device__ __forceinline__ void fft_inv_mfma(const C2<float> (&y)[E], C2<float> (&x)[E], C2<float>* b0, C2<float>* b1,
+                                             const C2<float>* tw, const uint16_t* dft16, const C2<float>* twp)
+{
+    const int tid = threadIdx.x;
+    v8bf fa[2][2];
+    load_dft16<+1>(dft16, fa);
+#pragma unroll
+    for (int r = 0; r < E; ++r) b0[pidx(tid + r * NT, E)] = y[r];
+    __syncthreads();
+    using Chain = MfmaChain<T, NT, E, T, +1, 0>;
+    Chain::run(b0, b1, tw, twp, fa, 1);
+    const C2<float>* src = Chain::out ? b1 : b0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) x[e] = src[pidx(tid * E + e, E)];
+    if constexpr (Chain::out == 0) __syncthreads();   // as in fft_fwd_acc
+}
+
+// fft_fwd_acc with the passes of MfmaChain (FP32 data, T >= 512): same
+// arguments and result layout
+template <int MODE, int T, int NT, int E, class F>
+__device__ __forceinline__ void fft_fwd_acc_mfma(const C2<float> (&x)[E], const F (&fac)[E], C2<float> (&out)[E],
+                                                 C2<float>* b0, C2<float>* b1, const C2<float>* tw, const void* table,
+                                                 const uint16_t* dft16, const C2<float>* twp)
+{
+    const int tid = threadIdx.x;
+    v8bf fa[2][2];
+    load_dft16<-1>(dft16, fa);
+#pragma unroll
+    for (int e = 0; e < E; ++e) b0[pidx(tid * E + e, E)] = cscale(x[e], (float)fac[e]);
+    __syncthreads();
+    using Chain = MfmaChain<T, NT, E, T, -1, 0>;
+    Chain::run(b0, b1, tw, twp, fa, 1);
+    const C2<float>* src = Chain::out ? b1 : b0;
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int w = tid + r * NT;
+        const C2<float> y = src[pidx(w, E)];
+        C2<float> g;
+        if constexpr (MODE == 0) {
+            g = y;
+        } else if constexpr (MODE == 1) {
+            g = cmul(((const C2<float>*)table)[w], y);
+        } else {
+            const float kk = ((const float*)table)[w];
+            g = C2<float>{kk * y.y, -kk * y.x};
+        }
+        out[r] = cadd(out[r], g);
+    }
+    // the next transform writes b0 first and passes a barrier before it writes b1
+    if constexpr (Chain::out == 0) __syncthreads();
+}
+
+// the forward transform of one RHS term: on MFMA for the mixed policy when
+// asked (mfma_fwd 1: the Kerr/Raman term; 2: every term), VALU otherwise
+template <int MODE, class Pol, int T, int NT, int E, class F>
+__device__ __forceinline__ void fft_fwd_term(const C2<typename Pol::R> (&x)[E], const F (&fac)[E],
+                                             C2<typename Pol::R> (&out)[E], C2<typename Pol::R>* b0,
+                                             C2<typename Pol::R>* b1, const C2<typename Pol::R>* tw, const void* table,
+                                             const NLArgs& a)
+{
+    if constexpr (Pol::mfma_fft && T >= 512) {
+        if (a.mfma_fwd >= (MODE == 2 ? 1 : 2)) {
+            fft_fwd_acc_mfma<MODE, T, NT, E>(x, fac, out, b0, b1, tw, table, (const uint16_t*)a.dft16,
+                                             (const C2<float>*)a.mfma_tw);
+            return;
+        }
+    }
+    fft_fwd_acc<MODE, typename Pol::R, T, NT, E>(x, fac, out, b0, b1, tw, table);
+}
+
 // ------------------------------------------------------------------ plasma
 struct PlasmaScratch {
     // laid out in a separate LDS area: [NT] W, [NT] c, [NT] rho (doubles), scan + reduction slots
@@ -318,9 +512,22 @@ __device__ __forceinline__ void nl_rhs(const C2<typename Pol::R> (&x)[E], C2<typ
     const int tid = threadIdx.x, n0 = tid * E;
     const C2<R>* tw = (const C2<R>*)a.tw;
 
-    // E(t) = IFFT(A)/N_T, chunk layout
+    // E(t) = IFFT(A)/N_T, chunk layout.  Mixed: on the matrix cores, kept when the
+    // whole line stays below the ionization threshold I = 1e-3 (with a 1e-4
+    // margin, far above the transform's error), else repeated in FP32
     C2<R> Ev[E];
-    fft_inv<R, T, NT, E>(x, Ev, sm.b0, sm.b1, tw);
+    bool have_e = false;
+    if constexpr (Pol::mfma_fft && T >= 512) {
+        if (a.mfma_inv && a.rho_init == 0.0) {
+            fft_inv_mfma<T, NT, E>(x, Ev, sm.b0, sm.b1, tw, (const uint16_t*)a.dft16, (const C2<float>*)a.mfma_tw);
+            const R s2 = (R)(a.inv_nt * a.inv_nt);
+            bool hot = false;
+#pragma unroll
+            for (int e = 0; e < E; ++e) hot |= (Ev[e].x * Ev[e].x + Ev[e].y * Ev[e].y) * s2 >= (R)0.9999e-3;
+            have_e = !__syncthreads_or(hot);
+        }
+    }
+    if (!have_e) fft_inv<R, T, NT, E>(x, Ev, sm.b0, sm.b1, tw);
     R lossf[E];
     R rho[E];
     bool anyloss = false, has_rho = false, ionizing;
@@ -354,8 +561,8 @@ __device__ __forceinline__ void nl_rhs(const C2<typename Pol::R> (&x)[E], C2<typ
 
 #pragma unroll
     for (int r = 0; r < E; ++r) out[r] = C2<R>{0, 0};
-    if (ionizing && __syncthreads_or(anyloss)) fft_fwd_acc<0, R, T, NT, E>(Ev, lossf, out, sm.b0, sm.b1, tw, nullptr);
-    if (has_rho) fft_fwd_acc<1, R, T, NT, E>(Ev, rho, out, sm.b0, sm.b1, tw, a.pf);
+    if (ionizing && __syncthreads_or(anyloss)) fft_fwd_term<0, Pol, T, NT, E>(Ev, lossf, out, sm.b0, sm.b1, tw, nullptr, a);
+    if (has_rho) fft_fwd_term<1, Pol, T, NT, E>(Ev, rho, out, sm.b0, sm.b1, tw, a.pf, a);
     if (!a.use_kerr) return;
 
     // Raman: s_n = z s_{n-1} + I_n; y = (1-fR) I + fR M Im(s (1 - z^T) + z^{n+1} s_{T-1})
@@ -389,7 +596,7 @@ __device__ __forceinline__ void nl_rhs(const C2<typename Pol::R> (&x)[E], C2<typ
 #pragma unroll
         for (int e = 0; e < E; ++e) yv[e] = Ev[e].x * Ev[e].x + Ev[e].y * Ev[e].y;
     }
-    fft_fwd_acc<2, R, T, NT, E>(Ev, yv, out, sm.b0, sm.b1, tw, a.kerr);
+    fft_fwd_term<2, Pol, T, NT, E>(Ev, yv, out, sm.b0, sm.b1, tw, a.kerr, a);
 }
 
 }  // namespace gs

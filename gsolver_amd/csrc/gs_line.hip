@@ -47,21 +47,17 @@ struct RK4Rows {
     static constexpr size_t bytes = in_lds ? row_bytes : 0;
 };
 
+// RK4 NL step of one radial line (the body of k_nl_rk4 and k_nl_rk4_batch)
 template <class Pol, int T, int E>
-__global__ __launch_bounds__((Shape<Pol, T, E>::NT), GS_NL_WAVES) void k_nl_rk4(const C2<typename Pol::R>* __restrict__ field,
-                                                                    C2<typename Pol::R>* __restrict__ b1, double dz,
-                                                                    NLArgs a, double* __restrict__ linemax)
+__device__ __forceinline__ void nl_rk4_line(const C2<typename Pol::R>* __restrict__ field, C2<typename Pol::R>* __restrict__ b1,
+                                            double dz, const NLArgs& a, double* __restrict__ linemax, int line,
+                                            unsigned char* smem_raw)
 {
     using R = typename Pol::R;
     constexpr int NT = Shape<Pol, T, E>::NT;
     constexpr bool ROWS_LDS = RK4Rows<Pol, T, E>::in_lds;
-    if (a.ctl) {
-        if (a.ctl->state != 0) return;
-        dz = a.ctl->dz;
-    }
-    extern __shared__ __align__(16) unsigned char smem_raw[];
     LineSmem<Pol, T, E>& sm = *reinterpret_cast<LineSmem<Pol, T, E>*>(smem_raw);
-    const int line = blockIdx.x, tid = threadIdx.x;
+    const int tid = threadIdx.x;
     const C2<R>* Ag = field + (size_t)line * T;
     C2<R>* b1row = b1 + (size_t)line * T;
     // A and the accumulator: LDS rows (FP32) or the global rows (FP64, the output row accumulates)
@@ -111,7 +107,33 @@ __global__ __launch_bounds__((Shape<Pol, T, E>::NT), GS_NL_WAVES) void k_nl_rk4(
         linemax[2 * line] = mA;
         linemax[2 * line + 1] = mN;
     }
-    if (flags) atomicOr(&a.status->flags, flags);
+    report_flags(a.status, flags);
+}
+
+template <class Pol, int T, int E>
+__global__ __launch_bounds__((Shape<Pol, T, E>::NT), GS_NL_WAVES) void k_nl_rk4(const C2<typename Pol::R>* __restrict__ field,
+                                                                    C2<typename Pol::R>* __restrict__ b1, double dz,
+                                                                    NLArgs a, double* __restrict__ linemax)
+{
+    if (a.ctl) {
+        if (a.ctl->state != 0) return;
+        dz = a.ctl->dz;
+    }
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    nl_rk4_line<Pol, T, E>(field, b1, dz, a, linemax, blockIdx.x, smem_raw);
+}
+
+// the NL step of several runs in one launch: block b is line items[b].y of run items[b].x
+template <class Pol, int T, int E>
+__global__ __launch_bounds__((Shape<Pol, T, E>::NT), GS_NL_WAVES) void k_nl_rk4_batch(const NLRun* __restrict__ runs,
+                                                                                 const int2* __restrict__ items)
+{
+    using R = typename Pol::R;
+    const int2 it = items[blockIdx.x];
+    const NLRun& r = runs[it.x];
+    if (r.a.ctl->state != 0) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    nl_rk4_line<Pol, T, E>((const C2<R>*)r.field, (C2<R>*)r.b1, r.a.ctl->dz, r.a, r.linemax, it.y, smem_raw);
 }
 
 template <class Pol, int T, int E>
@@ -130,7 +152,7 @@ __global__ __launch_bounds__((Shape<Pol, T, E>::NT)) void k_nl_response(const C2
     nl_rhs<Pol, T, E>(x, y, sm, a, &flags);
 #pragma unroll
     for (int r = 0; r < E; ++r) out[(size_t)line * T + tid + r * NT] = y[r];
-    if (flags) atomicOr(&a.status->flags, flags);
+    report_flags(a.status, flags);
 }
 
 // ------------------------------------------------------------------ line FFT / diagnostics
@@ -276,7 +298,7 @@ __global__ __launch_bounds__((Shape<Pol, T, E>::NT)) void k_line_diag(const C2<t
         if (maxI) maxI[line] = mi;
         if (rhomax) rhomax[line] = rm;
     }
-    if (flags) atomicOr(&a.status->flags, flags);
+    report_flags(a.status, flags);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -351,13 +373,49 @@ int launch_nl_rk4(Prec p, int nt, const void* field, void* b1, double dz, const 
 {
     if (p == kFP64) GS_T_SWITCH(nt, (nl_rk4_T<PolF64, T_>(field, b1, dz, a, linemax, st)));
     if (p == kFP32) GS_T_SWITCH(nt, (nl_rk4_T<PolF32, T_>(field, b1, dz, a, linemax, st)));
+    if (a.mfma_fwd || a.mfma_inv)
+        GS_T_SWITCH(nt, (nl_rk4_TE<PolMixMfma, T_, DefaultE<PolMixMfma, T_>::v>(field, b1, dz, a, linemax, st)));
     GS_T_SWITCH(nt, (nl_rk4_T<PolMix, T_>(field, b1, dz, a, linemax, st)));
+}
+
+template <class Pol, int T, int E>
+static int nl_rk4_batch_TE(const NLRun* runs, const int2* items, int nitems, hipStream_t st)
+{
+    const size_t sm = sizeof(LineSmem<Pol, T, E>) + RK4Rows<Pol, T, E>::bytes;
+    auto kf = k_nl_rk4_batch<Pol, T, E>;
+    HIPCHK(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    hipLaunchKernelGGL(kf, dim3(nitems), dim3(Shape<Pol, T, E>::NT), sm, st, runs, items);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// the line shape k_nl_rk4 uses for this (policy, N_T, nl_e): as nl_rk4_T
+template <class Pol, int T>
+static int nl_rk4_batch_T(int nl_e, const NLRun* runs, const int2* items, int nitems, hipStream_t st)
+{
+    if constexpr (sizeof(typename Pol::R) == 4 && T >= 1024) {
+        if (nl_e == 4) return nl_rk4_batch_TE<Pol, T, 4>(runs, items, nitems, st);
+        if (nl_e == 8) return nl_rk4_batch_TE<Pol, T, 8>(runs, items, nitems, st);
+        if (nl_e == 16) return nl_rk4_batch_TE<Pol, T, 16>(runs, items, nitems, st);
+    }
+    return nl_rk4_batch_TE<Pol, T, DefaultE<Pol, T>::v>(runs, items, nitems, st);
+}
+
+int launch_nl_rk4_batch(Prec p, int nt, int nl_e, bool mfma_fft, const NLRun* runs, const int2* items, int nitems,
+                        hipStream_t st)
+{
+    if (p == kFP64) GS_T_SWITCH(nt, (nl_rk4_batch_T<PolF64, T_>(nl_e, runs, items, nitems, st)));
+    if (p == kFP32) GS_T_SWITCH(nt, (nl_rk4_batch_T<PolF32, T_>(nl_e, runs, items, nitems, st)));
+    if (mfma_fft)
+        GS_T_SWITCH(nt, (nl_rk4_batch_TE<PolMixMfma, T_, DefaultE<PolMixMfma, T_>::v>(runs, items, nitems, st)));
+    GS_T_SWITCH(nt, (nl_rk4_batch_T<PolMix, T_>(nl_e, runs, items, nitems, st)));
 }
 
 int launch_nl_response(Prec p, int nt, const void* in, void* out, int nlines, const NLArgs& a, hipStream_t st)
 {
     if (p == kFP64) GS_T_SWITCH(nt, (nl_resp_T<PolF64, T_>(in, out, nlines, a, st)));
     if (p == kFP32) GS_T_SWITCH(nt, (nl_resp_T<PolF32, T_>(in, out, nlines, a, st)));
+    if (a.mfma_fwd || a.mfma_inv) GS_T_SWITCH(nt, (nl_resp_T<PolMixMfma, T_>(in, out, nlines, a, st)));
     GS_T_SWITCH(nt, (nl_resp_T<PolMix, T_>(in, out, nlines, a, st)));
 }
 

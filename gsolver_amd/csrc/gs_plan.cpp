@@ -103,11 +103,23 @@ struct gs_plan {
     DevBuf field, b1, b2, fk, tmp;       // [rp][T] in field precision; fk = FIELD_k = B2 on the
                                          // unmasked points (the live state), b2 = B2's masked points
     DevBuf amat, at64;                   // QDHT operators
-    DevBuf tw, tw64, zpow, pf, kerr, ionrate, k2, lrate, lgfac, avgw;
+    DevBuf tw, tw64, zpow, pf, kerr, ionrate, k2, lrate, lgfac, avgw, dft16, mfma_tw;
     DevBuf linemax, status, diag, et, rhot, scratch64a, scratch64b;
     std::vector<DevBuf> wfilt;
     gs::NLArgs nl{};
     gs::LstepArgs ls{};
+
+    // column skipping of the march's two QDHTs: the 32-column blocks that hold a
+    // live omega column (ctile) and the others (ztile); FIELD keeps values in the
+    // skipped blocks until a QDHT back clears them (dead_dirty)
+    DevBuf ctile, ztile;
+    int nct = 0, nzt = 0;
+    bool qskip = true;
+    bool dead_dirty = true;
+    gs::QCols qcols(bool zero_dead) const
+    {
+        return gs::QCols{ctile.as<int>(), nct, ztile.as<int>(), nzt, zero_dead ? 1 : 0};
+    }
 
     double ZSTEP = 0, CURRENT_Z = 0;
     long STEP_N = 0, n_reject = 0;
@@ -221,6 +233,29 @@ void upload_bf16_fragments(DevBuf& dst, const std::vector<double>& M, int rp)
     upload(dst, fr.data(), fr.size() * 2);
 }
 
+// The forward radix-16 DFT Y_q = sum_r x_r e^{-2 pi i q r/16} as the real 32 x 32
+// A operand of v_mfma_f32_32x32x16_bf16 (rows Re Y_0..15, Im Y_0..15; k slice 0
+// takes Re x_r, slice 1 Im x_r), split into bf16 hi = bf16(a), lo = bf16(a - hi),
+// in fragment order [k slice][hi, lo][lane][8]: lane = m + 32 (r / 8), element r % 8
+void upload_dft16_fragments(DevBuf& dst)
+{
+    std::vector<uint16_t> fr(2 * 2 * 2 * 64 * 8);
+    for (int sg = 0; sg < 2; ++sg)          // sign -1 (forward), +1 (inverse)
+        for (int sl = 0; sl < 2; ++sl)
+            for (int lane = 0; lane < 64; ++lane)
+                for (int e = 0; e < 8; ++e) {
+                    const int m = lane & 31, r = 8 * (lane >> 5) + e, q = m & 15;
+                    const double ang = 2 * M_PI * ((q * r) & 15) / 16.0;
+                    const double c = std::cos(ang), sn = (sg ? 1.0 : -1.0) * std::sin(ang);   // e^{+-i ang} = c + i sn
+                    // rows < 16: Re Y = C Re x - Sn Im x; rows >= 16: Im Y = Sn Re x + C Im x
+                    const double a = m < 16 ? (sl == 0 ? c : -sn) : (sl == 0 ? sn : c);
+                    const uint16_t h = bf16_rn_host((float)a), l = bf16_rn_host((float)a - bf16_to_float(h));
+                    fr[(((sg * 2 + sl) * 2 + 0) * 64 + lane) * 8 + e] = h;
+                    fr[(((sg * 2 + sl) * 2 + 1) * 64 + lane) * 8 + e] = l;
+                }
+    upload(dst, fr.data(), fr.size() * 2);
+}
+
 void build_plan(gs_plan* p)
 {
     const gs::Setup& s = p->s;
@@ -314,6 +349,19 @@ void build_plan(gs_plan* p)
             }
         }
         upload(p->k2, k2.data(), k2.size() * 8);
+        // 32-column blocks with at least one unmasked point: the columns the march's QDHTs compute
+        std::vector<int> ct, zt;
+        for (int c0 = 0; c0 < nt; c0 += 32) {
+            bool live = false;
+            for (int i = 0; i < nr && !live; ++i)
+                for (int w = c0; w < c0 + 32 && !live; ++w) live = g[(size_t)i * nt + w] >= 0;
+            (live ? ct : zt).push_back(c0);
+        }
+        p->nct = (int)ct.size();
+        p->nzt = (int)zt.size();
+        if (p->nct > 0) upload(p->ctile, ct.data(), ct.size() * sizeof(int));
+        if (p->nzt > 0) upload(p->ztile, zt.data(), zt.size() * sizeof(int));
+        p->qskip = p->nct > 0 && !std::getenv("GS_QDHT_ALL_COLUMNS");
         if (p->prec == gs::kFP32) {
             std::vector<float> rf(rate.begin(), rate.end());
             upload(p->lrate, rf.data(), rf.size() * 4);
@@ -376,6 +424,35 @@ void build_plan(gs_plan* p)
     a.pf = p->pf.p;
     a.kerr = p->kerr.p;
     a.tw = p->tw.p;
+    if (p->prec == gs::kMIXED) {
+        upload_dft16_fragments(p->dft16);
+        // lane-order twiddles of the MFMA radix-16 passes at Ns = 16, 256, ... (while 16 Ns
+        // divides N_T): [pass][group j][r] = e^{-2 pi i r (j mod Ns) / (16 Ns)}
+        std::vector<float> t;
+        for (int Ns = 16; nt % (16 * Ns) == 0; Ns *= 16)
+            for (int j = 0; j < nt / 16; ++j)
+                for (int r = 0; r < 16; ++r) {
+                    const size_t m = (size_t)r * (j & (Ns - 1)) * (nt / (16 * Ns));
+                    t.push_back((float)tw64[2 * m]);
+                    t.push_back((float)tw64[2 * m + 1]);
+                }
+        if (!t.empty()) upload(p->mfma_tw, t.data(), t.size() * 4);
+    }
+    a.dft16 = p->dft16.p;
+    a.mfma_tw = p->mfma_tw.p;
+    // which mixed line FFTs run on the matrix cores (GS_MIX_FFT: valu, the default; kerr: the
+    // Kerr/Raman forward transform; fwd: every forward transform; kerr_inv / all: those and
+    // the inverse transform of non-ionizing lines).  VALU is the faster and the more
+    // accurate here; the MFMA variants are kept for measurement.
+    a.mfma_fwd = a.mfma_inv = 0;
+    if (p->prec == gs::kMIXED) {
+        const char* m = std::getenv("GS_MIX_FFT");
+        const std::string mode = m ? m : "valu";
+        if (mode != "valu" && mode != "kerr" && mode != "fwd" && mode != "kerr_inv" && mode != "all")
+            throw gs::Error(GS_E_ARG, "GS_MIX_FFT must be valu, kerr, fwd, kerr_inv or all");
+        a.mfma_fwd = mode == "valu" ? 0 : (mode == "fwd" || mode == "all") ? 2 : 1;
+        a.mfma_inv = (mode == "kerr_inv" || mode == "all") ? 1 : 0;
+    }
     a.status = p->status.as<gs::DevStatus>();
     // samples per thread of the RK4 line kernel (tuning knob; 0 = built-in default)
     a.nl_e = 0;
@@ -493,16 +570,26 @@ void nl_step(gs_plan* p, double dz, double* maxA2, double* maxNL2)
     *maxNL2 = h.maxNL2;
 }
 
+// the two QDHTs of an accepted step: B1 -> k-space with the linear step fused
+// (FIELD_k), FIELD_k -> FIELD; with column skipping only the live blocks are
+// computed, and the first QDHT back after FIELD was set clears the others
+void enqueue_linear(gs_plan* p, const gs::LstepArgs& ls, hipEvent_t mid)
+{
+    const void* A = p->prec == gs::kFP64 ? p->at64.p : p->amat.p;
+    const gs::QCols fwd = p->qcols(false), back = p->qcols(p->dead_dirty);
+    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->b1.p, nullptr, &ls, p->st, nullptr, p->qskip ? &fwd : nullptr));
+    if (mid) HCK(hipEventRecord(mid, p->st));
+    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->fk.p, p->field.p, nullptr, p->st, nullptr,
+                        p->qskip ? &back : nullptr));
+    p->dead_dirty = false;
+}
+
 void linear_half(gs_plan* p, double dz)
 {
     p->ls.dz = dz;
     p->ls.seed = (uint32_t)p->STEP_N;
     HCK(hipEventRecord(p->ev[2], p->st));
-    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, p->prec == gs::kFP64 ? p->at64.p : p->amat.p, p->b1.p,
-                        nullptr, &p->ls, p->st));
-    HCK(hipEventRecord(p->ev[3], p->st));
-    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, p->prec == gs::kFP64 ? p->at64.p : p->amat.p, p->fk.p,
-                        p->field.p, nullptr, p->st));
+    enqueue_linear(p, p->ls, p->ev[3]);
 }
 
 void finish_timings(gs_plan* p, hipEvent_t end)
@@ -560,8 +647,10 @@ void enqueue_slot(gs_plan* p)
     gs::LstepArgs l = p->ls;
     l.ctl = c;
     const void* A = p->prec == gs::kFP64 ? p->at64.p : p->amat.p;
-    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->b1.p, nullptr, &l, p->st));
-    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->fk.p, p->field.p, nullptr, p->st, c));
+    // the QDHT back carries its clearing blocks; they act only when DevCtl::dead_dirty is set
+    const gs::QCols fwd = p->qcols(false), back = p->qcols(false);
+    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->b1.p, nullptr, &l, p->st, nullptr, p->qskip ? &fwd : nullptr));
+    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->fk.p, p->field.p, nullptr, p->st, c, p->qskip ? &back : nullptr));
     LCK(gs::launch_finish(c, p->st));
 }
 
@@ -587,28 +676,30 @@ void launch_slot(gs_plan* p)
     else enqueue_slot(p);
 }
 
+// the plan's host mirror <- a controller state read back from the device
+void pull_state(gs_plan* p, const gs::DevCtl& h)
+{
+    p->ZSTEP = h.dz;
+    p->CURRENT_Z = h.z;
+    p->STEP_N = (long)h.steps;
+    p->n_reject = (long)h.rejects;
+    p->dead_dirty = h.dead_dirty != 0;
+}
+
 // host mirror <- device controller state
 void pull_ctl(gs_plan* p)
 {
     HCK(hipMemcpyAsync(p->ctl_host, p->ctl.p, sizeof(gs::DevCtl), hipMemcpyDeviceToHost, p->st));
     sync(p);
-    const gs::DevCtl& h = *p->ctl_host;
-    p->ZSTEP = h.dz;
-    p->CURRENT_Z = h.z;
-    p->STEP_N = (long)h.steps;
-    p->n_reject = (long)h.rejects;
+    pull_state(p, *p->ctl_host);
 }
 
 // arm the controller for the plane ZNET[n_Z] from the host state
-void arm_ctl(gs_plan* p)
+// controller state for the plane ZNET[n_Z] from the plan's host state
+void fill_ctl(const gs_plan* p, gs::DevCtl& h)
 {
-    if (!p->ctl.p) {
-        p->ctl.alloc(sizeof(gs::DevCtl));
-        HCK(hipHostMalloc((void**)&p->ctl_host, sizeof(gs::DevCtl), 0));
-    }
     const auto& Z = p->s.ZNET;
     const int nz = std::max(p->n_Z, 1);
-    gs::DevCtl& h = *p->ctl_host;
     h = gs::DevCtl{};
     h.dz = p->ZSTEP;
     h.z = p->CURRENT_Z;
@@ -617,8 +708,19 @@ void arm_ctl(gs_plan* p)
     h.tol = p->s.MAX_TOLERANCE;
     h.steps = p->STEP_N;
     h.rejects = p->n_reject;
+    h.dead_dirty = p->dead_dirty ? 1 : 0;
+}
+
+void arm_ctl(gs_plan* p)
+{
+    if (!p->ctl.p) {
+        p->ctl.alloc(sizeof(gs::DevCtl));
+        HCK(hipHostMalloc((void**)&p->ctl_host, sizeof(gs::DevCtl), 0));
+    }
+    fill_ctl(p, *p->ctl_host);
     HCK(hipMemcpyAsync(p->ctl.p, p->ctl_host, sizeof(gs::DevCtl), hipMemcpyHostToDevice, p->st));
 }
+
 
 [[noreturn]] void ctl_error(gs_plan* p, int err)
 {
@@ -770,6 +872,142 @@ int guarded(F&& f)
 
 }  // namespace
 
+// ---- batched march: the step attempts of several runs, one launch per phase
+// (every run's NL lines; the controllers; every run's QDHT + linear step; every
+// run's QDHT back; the bookkeeping), captured once into a graph.  Runs that
+// sit at a plane or have failed are no-ops in it (their DevCtl::state).
+struct BatchMarch {
+    int n = 0;
+    hipStream_t st = nullptr;
+    DevBuf ctls, nlruns, ctlruns;
+    gs::DevCtl* host = nullptr;     // pinned mirror of the n controllers
+    struct NLGroup {
+        gs::Prec prec;
+        int nt, nl_e;
+        bool mfma;
+        DevBuf items;
+        int nitems;
+    };
+    struct GemmLaunch {
+        gs::Prec prec;
+        bool lstep;
+        DevBuf runs, brun;
+        int nblocks;
+    };
+    std::vector<NLGroup> nlg;
+    std::vector<GemmLaunch> gl;
+    hipGraphExec_t exec = nullptr;
+
+    BatchMarch(gs_plan* const* plans, int n_) : n(n_)
+    {
+        HCK(hipStreamCreate(&st));
+        ctls.alloc(sizeof(gs::DevCtl) * n);
+        HCK(hipHostMalloc((void**)&host, sizeof(gs::DevCtl) * n, 0));
+        std::memset(host, 0, sizeof(gs::DevCtl) * n);
+        gs::DevCtl* dctl = ctls.as<gs::DevCtl>();
+        std::vector<gs::NLRun> nr(n);
+        std::vector<gs::CtlRun> cr(n);
+        for (int i = 0; i < n; ++i) {
+            gs_plan* p = plans[i];
+            nr[i] = gs::NLRun{p->field.p, p->b1.p, p->linemax.as<double>(), p->nl};
+            nr[i].a.ctl = dctl + i;
+            cr[i] = gs::CtlRun{p->s.NR, p->linemax.as<double>(), p->status.as<gs::DevStatus>(), dctl + i};
+            // NL lines: one group per (precision, N_T, line shape)
+            const bool mfma = p->nl.mfma_fwd || p->nl.mfma_inv;
+            NLGroup* g = nullptr;
+            for (auto& x : nlg)
+                if (x.prec == p->prec && x.nt == p->s.NT && x.nl_e == p->nl.nl_e && x.mfma == mfma) g = &x;
+            if (!g) {
+                nlg.push_back(NLGroup{p->prec, p->s.NT, p->nl.nl_e, mfma, DevBuf(), 0});
+                g = &nlg.back();
+            }
+            g->nitems += p->s.NR;
+        }
+        upload(nlruns, nr.data(), sizeof(gs::NLRun) * n);
+        upload(ctlruns, cr.data(), sizeof(gs::CtlRun) * n);
+        for (auto& g : nlg) {
+            std::vector<int2> it;
+            for (int i = 0; i < n; ++i) {
+                gs_plan* p = plans[i];
+                const bool mfma = p->nl.mfma_fwd || p->nl.mfma_inv;
+                if (p->prec == g.prec && p->s.NT == g.nt && p->nl.nl_e == g.nl_e && mfma == g.mfma)
+                    for (int r = 0; r < p->s.NR; ++r) it.push_back(int2{i, r});
+            }
+            upload(g.items, it.data(), sizeof(int2) * it.size());
+        }
+        // QDHTs: one launch per precision and direction; each run's blocks start on a multiple of 8
+        for (gs::Prec pr : {gs::kFP64, gs::kFP32, gs::kMIXED})
+            for (bool lstep : {true, false}) {
+                std::vector<gs::GemmRun> runs;
+                std::vector<int> brun;
+                for (int i = 0; i < n; ++i) {
+                    gs_plan* p = plans[i];
+                    if (p->prec != pr) continue;
+                    gs::GemmRun g{};
+                    g.A = pr == gs::kFP64 ? p->at64.p : p->amat.p;
+                    g.X = lstep ? p->b1.p : p->fk.p;
+                    g.Y = lstep ? nullptr : p->field.p;
+                    g.K = p->rp;
+                    g.NC = p->s.NT;
+                    if (lstep) g.la = p->ls;
+                    g.la.ctl = dctl + i;
+                    if (p->qskip) g.qc = p->qcols(false);
+                    g.nb = gs::qdht_blocks(pr, p->rp, p->s.NT, p->qskip ? &g.qc : nullptr, !lstep, true);
+                    g.b0 = (int)brun.size();
+                    const int padded = (g.nb + 7) / 8 * 8;
+                    brun.insert(brun.end(), padded, (int)runs.size());
+                    runs.push_back(g);
+                }
+                if (runs.empty()) continue;
+                gl.push_back(GemmLaunch{pr, lstep, DevBuf(), DevBuf(), (int)brun.size()});
+                upload(gl.back().runs, runs.data(), sizeof(gs::GemmRun) * runs.size());
+                upload(gl.back().brun, brun.data(), sizeof(int) * brun.size());
+            }
+        // capture one batched attempt
+        hipGraph_t graph = nullptr;
+        HCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue();
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        HCK(hipStreamEndCapture(st, &graph));
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HCK(e);
+    }
+    BatchMarch(const BatchMarch&) = delete;
+    BatchMarch& operator=(const BatchMarch&) = delete;
+    ~BatchMarch()
+    {
+        if (st) (void)hipStreamSynchronize(st);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (host) (void)hipHostFree(host);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    void enqueue()
+    {
+        for (auto& g : nlg)
+            LCK(gs::launch_nl_rk4_batch(g.prec, g.nt, g.nl_e, g.mfma, nlruns.as<gs::NLRun>(), g.items.as<int2>(),
+                                        g.nitems, st));
+        LCK(gs::launch_controller_batch(ctlruns.as<gs::CtlRun>(), n, st));
+        for (bool lstep : {true, false})
+            for (auto& g : gl)
+                if (g.lstep == lstep)
+                    LCK(gs::launch_qdht_batch(g.prec, lstep, g.runs.as<gs::GemmRun>(), g.brun.as<int>(), g.nblocks, st));
+        LCK(gs::launch_finish_batch(ctls.as<gs::DevCtl>(), n, st));
+    }
+    void launch() { HCK(hipGraphLaunch(exec, st)); }
+    void send() { HCK(hipMemcpyAsync(ctls.p, host, sizeof(gs::DevCtl) * n, hipMemcpyHostToDevice, st)); }
+    void receive()
+    {
+        HCK(hipMemcpyAsync(host, ctls.p, sizeof(gs::DevCtl) * n, hipMemcpyDeviceToHost, st));
+        HCK(hipStreamSynchronize(st));
+    }
+};
+
 // ====================================================================== C-ABI
 extern "C" {
 
@@ -815,7 +1053,14 @@ int gs_dims(gs_plan* p, int* nr, int* nt, int* nz)
     return 0;
 }
 
+static int get_scalar(gs_plan* p, const char* name, double* v);
+
 int gs_get_scalar(gs_plan* p, const char* name, double* v)
+{
+    return guarded([&]() { return get_scalar(p, name, v); });
+}
+
+static int get_scalar(gs_plan* p, const char* name, double* v)
 {
     if (!p || !name || !v) return fail(GS_E_ARG, "null argument");
     const std::string n(name);
@@ -847,6 +1092,15 @@ int gs_get_scalar(gs_plan* p, const char* name, double* v)
     else if (n == "NEUTRAL_DENSITY") *v = s.neutral_density;
     else if (n == "qS") *v = s.qS;
     else if (n == "precision") *v = p->prec;
+    else if (n == "QDHT_SKIP_COLUMNS") *v = p->qskip ? 1 : 0;
+    else if (n == "SCAN_FALLBACKS") {
+        // line launches whose plasma scan needed the chunk-sequential fallback, since plan creation
+        gs::DevStatus h;
+        HCK(hipMemcpyAsync(&h, p->status.p, sizeof h, hipMemcpyDeviceToHost, p->st));
+        sync(p);
+        *v = h.fallbacks;
+    }
+    else if (n == "QDHT_LIVE_BLOCKS") *v = p->nct;
     else if (n == "WAVENUMBER0_re") *v = s.WAVENUMBER0.real();
     else return fail(GS_E_ARG, "unknown scalar " + n);
     return 0;
@@ -865,6 +1119,16 @@ int gs_set_scalar(gs_plan* p, const char* name, double v)
         if (p->n_Z == 0) p->dumps_started = false;   // a march from plane 0 starts its files again
     }
     else if (n == "MAX_TOLERANCE") p->s.MAX_TOLERANCE = v;
+    else if (n == "QDHT_SKIP_COLUMNS") {
+        // 1: the march's QDHTs compute only the 32-column blocks holding unmasked
+        // points; 0: every column (the step graph is captured again)
+        const bool on = v != 0 && p->nct > 0;
+        if (on != p->qskip && p->slot_exec) {
+            (void)hipGraphExecDestroy(p->slot_exec);
+            p->slot_exec = nullptr;
+        }
+        p->qskip = on;
+    }
     else if (n == "ADJOINT_CHECKPOINT_BYTES") {
         if (v != v) return fail(GS_E_ARG, "ADJOINT_CHECKPOINT_BYTES must be a number");
         p->adj_budget = v < 0 ? -1.0 : v;     // negative: 60 % of the free device memory
@@ -920,6 +1184,16 @@ static int copy_table(const gs::Setup& s, const char* name, double* out, size_t 
     return fail(GS_E_ARG, "unknown table " + nm);
 }
 
+// after a buffer was set through the API: FIELD_k is zero wherever the linear
+// step zeroes FIELD (the skipped QDHT columns rely on it), and a new FIELD may
+// hold values in the skipped columns until a QDHT back clears them
+static void mark_written(gs_plan* p, int buffer)
+{
+    if (buffer == GS_FK)
+        LCK(gs::launch_b2_split(p->prec, p->fk.p, p->fk.p, p->lgfac.p, (size_t)p->rp * p->s.NT, p->st));
+    if (buffer == GS_FIELD) p->dead_dirty = true;
+}
+
 int gs_get(gs_plan* p, int buffer, double* host)
 {
     return guarded([&]() {
@@ -947,6 +1221,7 @@ int gs_put(gs_plan* p, int buffer, const double* host)
         LCK(gs::launch_convert(p->prec, tmp.as<double>(), dst, n, 1, p->st));
         if (buffer == GS_B2)   // the unmasked part is the state the next linear step updates
             LCK(gs::launch_b2_split(p->prec, p->b2.p, p->fk.p, p->lgfac.p, (size_t)p->rp * p->s.NT, p->st));
+        mark_written(p, buffer);
         sync(p);
         return 0;
     });
@@ -1028,6 +1303,7 @@ int gs_qdht(gs_plan* p, int buffer)
             LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, b, p->tmp.p, nullptr, p->st));
             HCK(hipMemcpyAsync(b, p->tmp.p, p->field_bytes(), hipMemcpyDeviceToDevice, p->st));
         }
+        mark_written(p, buffer);
         sync(p);
         return 0;
     });
@@ -1139,68 +1415,53 @@ int gs_march_many(gs_plan* const* plans, int n, const char* const* outdirs, int*
 {
     return guarded([&]() {
         if (!plans || n <= 0) return fail(GS_E_ARG, "gs_march_many: no plans");
-        for (int i = 0; i < n; ++i)
-            if (!plans[i]) return fail(GS_E_ARG, "gs_march_many: null plan");
-        // every run keeps its own stream, controller and dumps; the host
-        // round-robins over the runs, enqueues a chunk of step attempts for each
-        // run that is not busy and polls the chunks' completion events, so the
-        // runs' kernels overlap on the device
-        constexpr long kChunk = 16;
-        struct Run {
-            DumpSet ds;
-            hipEvent_t done = nullptr;
-            bool busy = false, active = true;
-        };
-        std::vector<Run> runs(n);
-        struct Guard {
-            std::vector<Run>& r;
-            ~Guard() { for (auto& x : r) if (x.done) (void)hipEventDestroy(x.done); }
-        } guard{runs};
         for (int i = 0; i < n; ++i) {
-            open_dumps(plans[i], outdirs && outdirs[i] ? outdirs[i] : ".", &runs[i].ds);
-            HCK(hipEventCreateWithFlags(&runs[i].done, hipEventDisableTiming));
-            if (rcs) rcs[i] = 1;
+            if (!plans[i]) return fail(GS_E_ARG, "gs_march_many: null plan");
+            for (int j = 0; j < i; ++j)
+                if (plans[j] == plans[i]) return fail(GS_E_ARG, "gs_march_many: a plan is listed twice");
         }
+        BatchMarch b(plans, n);
+        std::vector<DumpSet> ds(n);
+        std::vector<char> active(n, 1);
         int left = n;
-        while (left > 0) {
-            bool progressed = false;
-            for (int i = 0; i < n; ++i) {
-                Run& r = runs[i];
-                gs_plan* p = plans[i];
-                if (!r.active) continue;
-                if (r.busy) {
-                    const hipError_t q = hipEventQuery(r.done);
-                    if (q == hipErrorNotReady) continue;
-                    HCK(q);
-                    r.busy = false;
-                    progressed = true;
-                    const gs::DevCtl& h = *p->ctl_host;
-                    p->ZSTEP = h.dz;
-                    p->CURRENT_Z = h.z;
-                    p->STEP_N = (long)h.steps;
-                    p->n_reject = (long)h.rejects;
-                    if (h.state == 2) ctl_error(p, h.err);
+        // the main loop's plane bookkeeping (solver_main.cpp:206-212) of run i; false once it is done
+        auto at_plane = [&](int i) {
+            gs_plan* p = plans[i];
+            const auto& Z = p->s.ZNET;
+            if (p->CURRENT_Z >= Z[p->n_Z]) {
+                do_dumps(p, &ds[i]);
+                if (p->n_Z == p->s.NZ - 1) {
+                    active[i] = 0;
+                    --left;
+                    if (rcs) rcs[i] = 0;
+                    return false;
                 }
-                const auto& Z = p->s.ZNET;
-                if (p->CURRENT_Z >= Z[p->n_Z]) {
-                    do_dumps(p, &r.ds);
-                    if (p->n_Z == p->s.NZ - 1) {
-                        r.active = false;
-                        --left;
-                        if (rcs) rcs[i] = 0;
-                        continue;
-                    }
-                    p->n_Z++;
-                }
-                if (p->n_Z == 0) throw gs::Error(GS_E_PHYSICS, "CURRENT_Z is below the first dump plane");
-                arm_ctl(p);
-                for (long k = 0; k < kChunk; ++k) launch_slot(p);
-                HCK(hipMemcpyAsync(p->ctl_host, p->ctl.p, sizeof(gs::DevCtl), hipMemcpyDeviceToHost, p->st));
-                HCK(hipEventRecord(r.done, p->st));
-                r.busy = true;
-                progressed = true;
+                p->n_Z++;
             }
-            if (!progressed) std::this_thread::yield();
+            if (p->n_Z == 0) throw gs::Error(GS_E_PHYSICS, "CURRENT_Z is below the first dump plane");
+            fill_ctl(p, b.host[i]);
+            return true;
+        };
+        for (int i = 0; i < n; ++i) {
+            if (rcs) rcs[i] = 1;
+            sync(plans[i]);
+            open_dumps(plans[i], outdirs && outdirs[i] ? outdirs[i] : ".", &ds[i]);
+        }
+        for (int i = 0; i < n; ++i) {
+            if (!at_plane(i)) b.host[i].state = 1;     // a run already at its last plane stays idle
+        }
+        constexpr int kChunk = 32;
+        while (left > 0) {
+            b.send();
+            for (int k = 0; k < kChunk; ++k) b.launch();
+            b.receive();
+            for (int i = 0; i < n; ++i) {
+                if (!active[i]) continue;
+                const gs::DevCtl h = b.host[i];
+                pull_state(plans[i], h);
+                if (h.state == 2) ctl_error(plans[i], h.err);
+                if (h.state == 1 && !at_plane(i)) b.host[i].state = 1;
+            }
         }
         return 0;
     });
@@ -1245,7 +1506,6 @@ int gs_fixed_steps(gs_plan* p, double dz, int n, double* ms4)
     return guarded([&]() {
         if (!p) return fail(GS_E_ARG, "null plan");
         if (n <= 0) return fail(GS_E_ARG, "gs_fixed_steps: n must be positive");
-        const void* A = p->prec == gs::kFP64 ? p->at64.p : p->amat.p;
         p->ls.dz = dz;
         if (!ms4) {
             // no timings asked for: the kernels alone, no event markers between them
@@ -1253,8 +1513,7 @@ int gs_fixed_steps(gs_plan* p, double dz, int n, double* ms4)
                 LCK(gs::launch_nl_rk4(p->prec, p->s.NT, p->field.p, p->b1.p, dz, p->nl, p->linemax.as<double>(), p->st));
                 LCK(gs::launch_reduce_max(p->s.NR, p->linemax.as<double>(), p->status.as<gs::DevStatus>(), p->st));
                 p->ls.seed = (uint32_t)(p->STEP_N + k);
-                LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->b1.p, nullptr, &p->ls, p->st));
-                LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->fk.p, p->field.p, nullptr, p->st));
+                enqueue_linear(p, p->ls, nullptr);
             }
             check_status(p, nullptr);
             for (int k = 0; k < n; ++k) p->CURRENT_Z += dz;
@@ -1276,9 +1535,7 @@ int gs_fixed_steps(gs_plan* p, double dz, int n, double* ms4)
             LCK(gs::launch_reduce_max(p->s.NR, p->linemax.as<double>(), p->status.as<gs::DevStatus>(), p->st));
             HCK(hipEventRecord(e[2], p->st));
             p->ls.seed = (uint32_t)(p->STEP_N + k);
-            LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->b1.p, nullptr, &p->ls, p->st));
-            HCK(hipEventRecord(e[3], p->st));
-            LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->fk.p, p->field.p, nullptr, p->st));
+            enqueue_linear(p, p->ls, e[3]);
             HCK(hipEventRecord(e[4], p->st));
         }
         check_status(p, nullptr);     // synchronises; NaN / blast flags of any of the steps
@@ -1382,6 +1639,7 @@ int gs_adjoint_forward(gs_plan* p, int n, const double* dz_rk, const double* dz_
         check_status(p, nullptr);
         p->adj_rk.assign(dz_rk, dz_rk + n);
         p->adj_seed0 = p->STEP_N;
+        p->dead_dirty = false;     // its QDHTs back transform every column
         p->adj_lin.assign(dz_lin, dz_lin + n);
         for (int k = 0; k < n; ++k) p->CURRENT_Z += dz_lin[k];
         p->ZSTEP = dz_lin[n - 1];

@@ -19,69 +19,101 @@
 namespace gs {
 
 // ------------------------------------------------------------------ QDHT GEMMs
-typedef float v4f __attribute__((ext_vector_type(4)));
-typedef float v16f __attribute__((ext_vector_type(16)));
 typedef double v4d __attribute__((ext_vector_type(4)));
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // blockIdx -> tile, grouping all row tiles of a column tile on one XCD so the
 // B columns are reused from that XCD's L2 (dispatch is round-robin over 8 XCDs)
-__device__ __forceinline__ void tile_of(int mt, int ntl, int* mi, int* ni)
+__device__ __forceinline__ void tile_of(int b, int mt, int ntl, int* mi, int* ni)
 {
-    const int tiles = mt * ntl, b = blockIdx.x;
+    const int tiles = mt * ntl;
     int t = b;
     if ((tiles & 7) == 0) t = (b & 7) * (tiles >> 3) + (b >> 3);
     *mi = t % mt;
     *ni = t / mt;
 }
 
-// FP32: tile 128 rows x 64 complex columns, K step 16, 4 waves (2x2)
-template <bool LSTEP>
-__global__ __launch_bounds__(256) void k_qdht_f32(const float* __restrict__ At, const C2<float>* __restrict__ X,
-                                                  C2<float>* __restrict__ Y, int K, int NC, LstepArgs la)
+// First column of column tile ni: the listed 32-column block of a
+// column-skipping transform, or ni * BNC for the full transform.
+__device__ __forceinline__ int col0_of(const QCols& qc, int ni, int BNC) { return qc.ctile ? qc.ctile[ni] : ni * BNC; }
+
+// Blocks past the computed tiles of a column-skipping QDHT back: block z
+// clears rows [m0, m0 + BM) of up to 8 skipped 32-column blocks of Y, when
+// asked to (zero_dead, or the device controller's dead_dirty of an accepted
+// attempt; the caller has already returned for a rejected one).
+template <class T>
+__device__ __forceinline__ void zero_skipped(T* __restrict__ Y, int NC, int mt, int BM, int z, const QCols& qc,
+                                             const DevCtl* ctl)
 {
-    constexpr int BM = 128, BNC = 64, BK = 16, PA = 32, PB = 32;
-    if (la.ctl) {
-        if (la.ctl->state != 0 || !la.ctl->accept) return;
-        la.dz = la.ctl->dz_lin;
-        la.seed = (uint32_t)la.ctl->steps;
+    if (!(ctl ? ctl->dead_dirty : qc.zero_dead)) return;
+    const int m0 = (z % mt) * BM, j0 = (z / mt) * 8, j1 = min(j0 + 8, qc.nzt);
+    for (int j = j0; j < j1; ++j) {
+        const int c0 = qc.ztile[j];
+        for (int i = threadIdx.x; i < BM * 32; i += blockDim.x) Y[(size_t)(m0 + i / 32) * NC + c0 + (i & 31)] = T{0, 0};
+    }
+}
+
+// FP32: tile 128 rows x BNC complex columns, K step 16, 4 waves: 2 x 2 waves
+// of 64 rows x 32 columns (BNC = 64, the full transform) or 4 x 1 waves of
+// 32 rows x 32 columns (BNC = 32, the column-skipping transform: ~900 live
+// columns at N_T = 4096 still give 29 x 8 tiles).  Each output sums its k
+// terms in the same order at either width, so the two agree bit for bit.
+template <bool LSTEP, int BNC>
+__device__ __forceinline__ void qdht_f32_body(int bid, const float* __restrict__ At, const C2<float>* __restrict__ X,
+                                              C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, const QCols& qc)
+{
+    constexpr int BM = 128, BK = 16, PA = 32, PB = 32;
+    constexpr int RB = BNC == 64 ? 2 : 1;      // 32-row blocks per wave
+    static_assert(BNC == 64 || BNC == 32, "tile width");
+    const DevCtl* ctl = la.ctl;
+    if (ctl) {
+        if (ctl->state != 0 || !ctl->accept) return;
+        la.dz = ctl->dz_lin;
+        la.seed = (uint32_t)ctl->steps;
     }
     __shared__ float As[BK][BM + PA];
     __shared__ float Br[BK][BNC + PB];
     __shared__ float Bi[BK][BNC + PB];
     const int M = K;
+    const int mt = M / BM, nct = qc.ctile ? qc.nct : NC / BNC;
+    if (bid >= mt * nct) {
+        if constexpr (!LSTEP) zero_skipped(Y, NC, mt, BM, bid - mt * nct, qc, ctl);
+        return;
+    }
     int mi, ni;
-    tile_of(M / BM, NC / BNC, &mi, &ni);
-    const int m0 = mi * BM, n0 = ni * BNC;
+    tile_of(bid, mt, nct, &mi, &ni);
+    const int m0 = mi * BM, n0 = col0_of(qc, ni, BNC);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int wm = (w >> 1) * 64, wn = (w & 1) * 32;
+    const int wm = BNC == 64 ? (w >> 1) * 64 : w * 32, wn = BNC == 64 ? (w & 1) * 32 : 0;
     const int lr = tid >> 4, lc = (tid & 15);
 
-    v16f acc[2][2];
+    v16f acc[RB][2];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < RB; ++a)
 #pragma unroll
         for (int c = 0; c < 2; ++c)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[a][c][r] = 0.f;
 
-    float4 ra0, ra1, rb0, rb1;
+    constexpr int NB4 = BNC / 32;       // float4 (two complex values) of B per thread
+    float4 ra0, ra1, rb[NB4];
     auto gload = [&](int k0) {
         const float* pa = At + (size_t)(k0 + lr) * M + m0 + lc * 8;
         ra0 = *(const float4*)pa;
         ra1 = *(const float4*)(pa + 4);
-        const float* pb = (const float*)(X + (size_t)(k0 + lr) * NC + n0 + lc * 4);
-        rb0 = *(const float4*)pb;
-        rb1 = *(const float4*)(pb + 4);
+        const float* pb = (const float*)(X + (size_t)(k0 + lr) * NC + n0 + lc * 2 * NB4);
+#pragma unroll
+        for (int j = 0; j < NB4; ++j) rb[j] = *(const float4*)(pb + 4 * j);
     };
     auto sstore = [&]() {
         *(float4*)&As[lr][lc * 8] = ra0;
         *(float4*)&As[lr][lc * 8 + 4] = ra1;
-        Br[lr][lc * 4 + 0] = rb0.x; Bi[lr][lc * 4 + 0] = rb0.y;
-        Br[lr][lc * 4 + 1] = rb0.z; Bi[lr][lc * 4 + 1] = rb0.w;
-        Br[lr][lc * 4 + 2] = rb1.x; Bi[lr][lc * 4 + 2] = rb1.y;
-        Br[lr][lc * 4 + 3] = rb1.z; Bi[lr][lc * 4 + 3] = rb1.w;
+#pragma unroll
+        for (int j = 0; j < NB4; ++j) {
+            const int c = lc * 2 * NB4 + 2 * j;
+            Br[lr][c] = rb[j].x; Bi[lr][c] = rb[j].y;
+            Br[lr][c + 1] = rb[j].z; Bi[lr][c + 1] = rb[j].w;
+        }
     };
     gload(0);
     sstore();
@@ -92,12 +124,13 @@ __global__ __launch_bounds__(256) void k_qdht_f32(const float* __restrict__ At, 
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 2) {
             const int kr = kk + (lane >> 5), cl = lane & 31;
-            const float a0 = As[kr][wm + cl], a1 = As[kr][wm + 32 + cl];
             const float br = Br[kr][wn + cl], bi = Bi[kr][wn + cl];
-            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, br, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, bi, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, br, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bi, acc[1][1], 0, 0, 0);
+#pragma unroll
+            for (int rb2 = 0; rb2 < RB; ++rb2) {
+                const float a = As[kr][wm + 32 * rb2 + cl];
+                acc[rb2][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, br, acc[rb2][0], 0, 0, 0);
+                acc[rb2][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bi, acc[rb2][1], 0, 0, 0);
+            }
         }
         __syncthreads();
         if (more) {
@@ -108,29 +141,23 @@ __global__ __launch_bounds__(256) void k_qdht_f32(const float* __restrict__ At, 
     const int col = n0 + wn + (lane & 31);
     const float amp = LSTEP ? lstep_amp<PolF32>(col, la) : 1.f;
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
+    for (int rb2 = 0; rb2 < RB; ++rb2)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int row = m0 + wm + rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int row = m0 + wm + rb2 * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             const size_t o = (size_t)row * NC + col;
-            if (LSTEP) lstep_point<PolF32>(o, acc[rb][0][r], acc[rb][1][r], amp, la);
-            else Y[o] = C2<float>{acc[rb][0][r], acc[rb][1][r]};
+            if (LSTEP) lstep_point<PolF32>(o, acc[rb2][0][r], acc[rb2][1][r], amp, la);
+            else Y[o] = C2<float>{acc[rb2][0][r], acc[rb2][1][r]};
         }
 }
-
-// BF16 3-pass: same tile, K step 32, operands k-contiguous in LDS.
-// split2: two floats -> packed bf16 hi pair and lo pair (round to nearest
-// even, v_cvt_pk_bf16_f32), hi + lo = v to about 2^-17 relative
-typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
-typedef float f32x2_t __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void split2(float v0, float v1, uint32_t* hi, uint32_t* lo)
+template <bool LSTEP, int BNC>
+__global__ __launch_bounds__(256) void k_qdht_f32(const float* __restrict__ At, const C2<float>* __restrict__ X,
+                                                  C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, QCols qc)
 {
-    const f32x2_t v = {v0, v1};
-    const bf16x2_t h = __builtin_convertvector(v, bf16x2_t);
-    const bf16x2_t l = __builtin_convertvector(v - __builtin_convertvector(h, f32x2_t), bf16x2_t);
-    *hi = __builtin_bit_cast(uint32_t, h);
-    *lo = __builtin_bit_cast(uint32_t, l);
+    qdht_f32_body<LSTEP, BNC>(blockIdx.x, At, X, Y, K, NC, la, qc);
 }
+
+// BF16 3-pass: same tile, K step 32, operands k-contiguous in LDS (split2 in gs_dev.h).
 
 // BF16 3-pass, large tile: 128 rows x 128 complex columns per workgroup,
 // 4 waves of 64 x 64 complex, K step 32 (two MFMA k-slices of 16).
@@ -210,7 +237,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 
     const int M = K;
     int mi, ni;
-    tile_of(M / BM, NC / BNC, &mi, &ni);
+    tile_of(blockIdx.x, M / BM, NC / BNC, &mi, &ni);
     const int m0 = mi * BM, n0 = ni * BNC;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
@@ -344,24 +371,207 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 }
 constexpr size_t kBf16T128Smem = t128::SMEM;
 
+// BF16 3-pass, narrow tile of the column-skipping transforms: 128 rows x 32
+// complex columns per workgroup, 4 waves of 32 rows x 32 columns, K step 32
+// (two MFMA k-slices of 16).  At N_T = 4096 about 900 of the columns are live,
+// which gives 29 x 8 tiles for 256 CUs where 128-column tiles would give 64.
+// The operator fragments are the t128 ones (each wave reads its 32-row block
+// from L2 into registers one K step ahead); B is split once per workgroup
+// into bf16 hi/lo planes in a double-buffered LDS tile shared by the four
+// waves.  Every output takes its MFMA terms in the t128 order (per k slice
+// lo*hi, hi*lo, hi*hi), so the two kernels agree bit for bit.
+namespace t32 {
+constexpr int BM = 128, BNC = 32, BK = 32, LD = BK + 8;
+constexpr int PLANE = BNC * LD;         // one [32 columns][LD] plane
+constexpr int BUF = 4 * PLANE;          // B hi re, B hi im, B lo re, B lo im
+// global loads of one K step: 4 consecutive k rows of one column
+__device__ __forceinline__ void gload_b(const C2<float>* gX, int NC, int step, int nk, float (&xr)[4], float (&xi)[4])
+{
+    step = step < nk ? step : nk - 1;      // past the end: reload the last step (never used)
+    const C2<float>* xp = gX + (size_t)step * BK * NC;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const C2<float> v = xp[(size_t)j * NC];
+        xr[j] = v.x;
+        xi[j] = v.y;
+    }
+}
+// the wave's A fragments of one K step: [kk][hi/lo] of its 32-row block
+__device__ __forceinline__ void gload_a(const uint16_t* A, int mi, int nk, int step, int rbb, int lane, v8bf (&a)[2][2])
+{
+    step = step < nk ? step : nk - 1;
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+        for (int hl = 0; hl < 2; ++hl) a[kk][hl] = *(const v8bf*)(A + t128::afrag(mi, nk, step, kk, rbb, hl) + lane * 8);
+}
+__device__ __forceinline__ void sstore_b(uint16_t* b, int oB, const float (&xr)[4], const float (&xi)[4])
+{
+    uint32_t hr[2], lr[2], hi[2], li[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        split2(xr[2 * j], xr[2 * j + 1], &hr[j], &lr[j]);
+        split2(xi[2 * j], xi[2 * j + 1], &hi[j], &li[j]);
+    }
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    *(u32x2*)(b + oB) = u32x2{hr[0], hr[1]};
+    *(u32x2*)(b + oB + PLANE) = u32x2{hi[0], hi[1]};
+    *(u32x2*)(b + oB + 2 * PLANE) = u32x2{lr[0], lr[1]};
+    *(u32x2*)(b + oB + 3 * PLANE) = u32x2{li[0], li[1]};
+}
+}  // namespace t32
+
+template <bool LSTEP>
+__device__ __forceinline__ void qdht_t32_body(int bid, const uint16_t* __restrict__ Afr, const C2<float>* __restrict__ X,
+                                              C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, const QCols& qc)
+{
+    using namespace t32;
+    __shared__ __align__(16) uint16_t sm[2 * BUF];
+    const DevCtl* ctl = la.ctl;
+    if (ctl) {
+        if (ctl->state != 0 || !ctl->accept) return;
+        la.dz = ctl->dz_lin;
+        la.seed = (uint32_t)ctl->steps;
+    }
+    const int M = K;
+    const int mt = M / BM, nct = qc.ctile ? qc.nct : NC / BNC;
+    if (bid >= mt * nct) {
+        if constexpr (!LSTEP) zero_skipped(Y, NC, mt, BM, bid - mt * nct, qc, ctl);
+        return;
+    }
+    int mi, ni;
+    tile_of(bid, mt, nct, &mi, &ni);
+    const int m0 = mi * BM, n0 = col0_of(qc, ni, BNC);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int bcol = tid & 31, bq = tid >> 5;               // B loader: column, k quarter (4 rows)
+    const int oB = bcol * LD + 4 * bq;                       // + plane * PLANE
+    const int rB = (lane & 31) * LD + 8 * (lane >> 5);      // + plane * PLANE + 16 kk
+    const C2<float>* gX = X + (size_t)(4 * bq) * NC + n0 + bcol;
+    const int nk = K / BK;
+
+    v16f acc[2];    // [re, im]
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[a][r] = 0.f;
+
+    struct OpsB {
+        v8bf h[2][2], l[2][2];   // [kk][pt]
+    };
+    auto read_b = [&](const uint16_t* b, OpsB& o) {
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt) {
+                o.h[kk][pt] = *(const v8bf*)(b + rB + pt * PLANE + 16 * kk);
+                o.l[kk][pt] = *(const v8bf*)(b + rB + (2 + pt) * PLANE + 16 * kk);
+            }
+    };
+    auto mfma_ops = [&](const v8bf (&a)[2][2], const OpsB& o) {
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt) acc[pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[kk][1], o.h[kk][pt], acc[pt], 0, 0, 0);
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt) acc[pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[kk][0], o.l[kk][pt], acc[pt], 0, 0, 0);
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt) acc[pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[kk][0], o.h[kk][pt], acc[pt], 0, 0, 0);
+        }
+    };
+    // interleave the 12 MFMAs of a step with the split VALU, the LDS writes and the global loads
+    auto schedule = [&]() {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);   // VALU
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);   // DS write
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x020, 2, 0);   // VMEM read
+        }
+    };
+
+    uint16_t* const buf0 = sm;
+    uint16_t* const buf1 = sm + BUF;
+    float xr[4], xi[4], nxr[4], nxi[4];
+    v8bf a0[2][2], a1[2][2];
+    gload_b(gX, NC, 0, nk, xr, xi);
+    sstore_b(buf0, oB, xr, xi);
+    gload_b(gX, NC, 1, nk, xr, xi);
+    gload_a(Afr, mi, nk, 0, w, lane, a0);
+    __syncthreads();
+    OpsB ops;
+    for (int k = 0; k < nk; k += 2) {
+        read_b(buf0, ops);
+        __builtin_amdgcn_sched_barrier(0);
+        gload_b(gX, NC, k + 2, nk, nxr, nxi);
+        gload_a(Afr, mi, nk, k + 1, w, lane, a1);
+        mfma_ops(a0, ops);
+        sstore_b(buf1, oB, xr, xi);
+        schedule();
+        __syncthreads();
+        read_b(buf1, ops);
+        __builtin_amdgcn_sched_barrier(0);
+        gload_b(gX, NC, k + 3, nk, xr, xi);
+        gload_a(Afr, mi, nk, k + 2, w, lane, a0);
+        mfma_ops(a1, ops);
+        sstore_b(buf0, oB, nxr, nxi);
+        schedule();
+        __syncthreads();
+    }
+    const int col = n0 + (lane & 31);
+    const double amp = LSTEP ? lstep_amp<PolMix>(col, la) : 1.0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        size_t o[8];
+        float yr[8], yi[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int r = 8 * h + j;
+            const int row = m0 + 32 * w + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            o[j] = (size_t)row * NC + col;
+            yr[j] = acc[0][r];
+            yi[j] = acc[1][r];
+        }
+        if constexpr (LSTEP) {
+            lstep_batch<PolMix, 8>(o, yr, yi, amp, la);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) Y[o[j]] = C2<float>{yr[j], yi[j]};
+        }
+    }
+}
+template <bool LSTEP>
+__global__ __launch_bounds__(256) void k_qdht_bf16x3_t32(const uint16_t* __restrict__ Afr, const C2<float>* __restrict__ X,
+                                                         C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, QCols qc)
+{
+    qdht_t32_body<LSTEP>(blockIdx.x, Afr, X, Y, K, NC, la, qc);
+}
+
 // FP64: tile 64 rows x 32 complex columns, K step 16, 4 waves (2x2)
 template <bool LSTEP>
-__global__ __launch_bounds__(256) void k_qdht_f64(const double* __restrict__ At, const C2<double>* __restrict__ X,
-                                                  C2<double>* __restrict__ Y, int K, int NC, LstepArgs la)
+__device__ __forceinline__ void qdht_f64_body(int bid, const double* __restrict__ At, const C2<double>* __restrict__ X,
+                                              C2<double>* __restrict__ Y, int K, int NC, LstepArgs la, const QCols& qc)
 {
     constexpr int BM = 64, BNC = 32, BK = 16, PA = 8, PB = 8;
-    if (la.ctl) {
-        if (la.ctl->state != 0 || !la.ctl->accept) return;
-        la.dz = la.ctl->dz_lin;
-        la.seed = (uint32_t)la.ctl->steps;
+    const DevCtl* ctl = la.ctl;
+    if (ctl) {
+        if (ctl->state != 0 || !ctl->accept) return;
+        la.dz = ctl->dz_lin;
+        la.seed = (uint32_t)ctl->steps;
     }
     __shared__ double As[BK][BM + PA];
     __shared__ double Br[BK][BNC + PB];
     __shared__ double Bi[BK][BNC + PB];
     const int M = K;
+    const int mt = M / BM, nct = qc.ctile ? qc.nct : NC / BNC;
+    if (bid >= mt * nct) {
+        if constexpr (!LSTEP) zero_skipped(Y, NC, mt, BM, bid - mt * nct, qc, ctl);
+        return;
+    }
     int mi, ni;
-    tile_of(M / BM, NC / BNC, &mi, &ni);
-    const int m0 = mi * BM, n0 = ni * BNC;
+    tile_of(bid, mt, nct, &mi, &ni);
+    const int m0 = mi * BM, n0 = col0_of(qc, ni, BNC);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = (w >> 1) * 32, wn = (w & 1) * 16;
     const int lr = tid >> 4, lc = tid & 15;
@@ -422,6 +632,12 @@ __global__ __launch_bounds__(256) void k_qdht_f64(const double* __restrict__ At,
             else Y[o] = C2<double>{acc[rb][0][r], acc[rb][1][r]};
         }
 }
+template <bool LSTEP>
+__global__ __launch_bounds__(256) void k_qdht_f64(const double* __restrict__ At, const C2<double>* __restrict__ X,
+                                                  C2<double>* __restrict__ Y, int K, int NC, LstepArgs la, QCols qc)
+{
+    qdht_f64_body<LSTEP>(blockIdx.x, At, X, Y, K, NC, la, qc);
+}
 
 // ------------------------------------------------------------------ small kernels
 __global__ void k_reduce_max(int nr, const double* __restrict__ lm, DevStatus* st)
@@ -444,7 +660,7 @@ __global__ void k_reduce_max(int nr, const double* __restrict__ lm, DevStatus* s
 // attempt counts in STEP_N and n_reject), dz_new = 0.5 tol sqrt(maxA2 / maxNL2),
 // collapse below dz_floor, clamp to the plane; the NaN / blast status of the
 // attempt ends the run as the reference's throws do
-__global__ void k_controller(int nr, const double* __restrict__ lm, DevStatus* status, DevCtl* ctl)
+__device__ __forceinline__ void controller_body(int nr, const double* __restrict__ lm, DevStatus* status, DevCtl* ctl)
 {
     __shared__ double red[64];
     if (ctl->state != 0) return;
@@ -478,16 +694,102 @@ __global__ void k_controller(int nr, const double* __restrict__ lm, DevStatus* s
     ctl->dz_lin = dzn;
     ctl->accept = repeat ? 0 : 1;
 }
+__global__ void k_controller(int nr, const double* __restrict__ lm, DevStatus* status, DevCtl* ctl)
+{
+    controller_body(nr, lm, status, ctl);
+}
 
 // after the linear half: CURRENT_Z += ZSTEP, STEP_N++ (solver_main.cpp:217-220);
 // stop at the plane
-__global__ void k_finish(DevCtl* ctl)
+__device__ __forceinline__ void finish_body(DevCtl* ctl)
 {
     if (ctl->state != 0 || !ctl->accept) return;
+    ctl->dead_dirty = 0;     // the accepted attempt's QDHT back has cleared the skipped columns
     ctl->z += ctl->dz_lin;
     ctl->steps += 1;
     ctl->accepted += 1;
     if (ctl->z >= ctl->z_plane) ctl->state = 1;
+}
+__global__ void k_finish(DevCtl* ctl) { finish_body(ctl); }
+
+// ------------------------------------------------------------------ batched march
+// Phases of the step attempts of several runs in one launch each (gs_march_many):
+// block b works for run brun[b] as that run's block b - b0; each run's block
+// range starts on a multiple of 8, so tile_of's XCD grouping holds within a
+// run, and blocks past a run's count (padding) return at once.  The bodies
+// are those of the single-run kernels, so every run's results are bit for bit
+// those of its own march.
+template <bool LSTEP>
+__global__ __launch_bounds__(256) void k_qdht_t32_batch(const GemmRun* __restrict__ runs, const int* __restrict__ brun)
+{
+    const GemmRun& g = runs[brun[blockIdx.x]];
+    const int bid = (int)blockIdx.x - g.b0;
+    if (bid >= g.nb) return;
+    qdht_t32_body<LSTEP>(bid, (const uint16_t*)g.A, (const C2<float>*)g.X, (C2<float>*)g.Y, g.K, g.NC, g.la, g.qc);
+}
+template <bool LSTEP>
+__global__ __launch_bounds__(256) void k_qdht_f32_batch(const GemmRun* __restrict__ runs, const int* __restrict__ brun)
+{
+    const GemmRun& g = runs[brun[blockIdx.x]];
+    const int bid = (int)blockIdx.x - g.b0;
+    if (bid >= g.nb) return;
+    qdht_f32_body<LSTEP, 32>(bid, (const float*)g.A, (const C2<float>*)g.X, (C2<float>*)g.Y, g.K, g.NC, g.la, g.qc);
+}
+template <bool LSTEP>
+__global__ __launch_bounds__(256) void k_qdht_f64_batch(const GemmRun* __restrict__ runs, const int* __restrict__ brun)
+{
+    const GemmRun& g = runs[brun[blockIdx.x]];
+    const int bid = (int)blockIdx.x - g.b0;
+    if (bid >= g.nb) return;
+    qdht_f64_body<LSTEP>(bid, (const double*)g.A, (const C2<double>*)g.X, (C2<double>*)g.Y, g.K, g.NC, g.la, g.qc);
+}
+__global__ void k_controller_batch(const CtlRun* __restrict__ runs)
+{
+    const CtlRun& r = runs[blockIdx.x];
+    controller_body(r.nr, r.linemax, r.status, r.ctl);
+}
+__global__ void k_finish_batch(DevCtl* ctls, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) finish_body(ctls + i);
+}
+
+int qdht_blocks(Prec p, int rp, int nt, const QCols* cols, bool back, bool gated)
+{
+    const int bm = p == kFP64 ? 64 : 128, mt = rp / bm;
+    if (!cols) return mt * (nt / 32);
+    return mt * cols->nct + ((back && (cols->zero_dead || gated)) ? mt * ((cols->nzt + 7) / 8) : 0);
+}
+
+int launch_qdht_batch(Prec p, bool lstep, const GemmRun* runs, const int* brun, int nblocks, hipStream_t st)
+{
+    const dim3 grid(nblocks), blk(256);
+    if (p == kFP64) {
+        if (lstep) hipLaunchKernelGGL(k_qdht_f64_batch<true>, grid, blk, 0, st, runs, brun);
+        else hipLaunchKernelGGL(k_qdht_f64_batch<false>, grid, blk, 0, st, runs, brun);
+    } else if (p == kFP32) {
+        if (lstep) hipLaunchKernelGGL(k_qdht_f32_batch<true>, grid, blk, 0, st, runs, brun);
+        else hipLaunchKernelGGL(k_qdht_f32_batch<false>, grid, blk, 0, st, runs, brun);
+    } else {
+        if (lstep) hipLaunchKernelGGL(k_qdht_t32_batch<true>, grid, blk, 0, st, runs, brun);
+        else hipLaunchKernelGGL(k_qdht_t32_batch<false>, grid, blk, 0, st, runs, brun);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int launch_controller_batch(const CtlRun* runs, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_controller_batch, dim3(n), dim3(1024), 0, st, runs);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int launch_finish_batch(DevCtl* ctls, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_finish_batch, dim3((n + 63) / 64), dim3(64), 0, st, ctls, n);
+    HIPCHK(hipGetLastError());
+    return 0;
 }
 
 template <class R>
@@ -536,29 +838,52 @@ __global__ void k_avg_spectrum(int nr, int nt, const C2<R>* __restrict__ f, cons
 }
 
 int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* in, void* out,
-                const LstepArgs* ls, hipStream_t st, const DevCtl* gate)
+                const LstepArgs* ls, hipStream_t st, const DevCtl* gate, const QCols* cols)
 {
     LstepArgs la = ls ? *ls : LstepArgs{};
     if (!ls) la.ctl = gate;
+    QCols qc = cols ? *cols : QCols{};
+    // blocks of a column-skipping QDHT back that clear the skipped columns: present when the
+    // host asks for it, or always in the device-driven march (they read DevCtl::dead_dirty)
+    auto zblocks = [&](int mt) {
+        if (!cols || ls || !(qc.zero_dead || gate)) return 0;
+        return mt * ((qc.nzt + 7) / 8);
+    };
+    if (cols && (nt % 32 || qc.nct < 1)) return -1001;
     if (p == kFP64) {
         if (rp % 64 || nt % 32) return -1001;
-        const dim3 grid((rp / 64) * (nt / 32));
-        if (ls) hipLaunchKernelGGL(k_qdht_f64<true>, grid, dim3(256), 0, st, (const double*)amat, (const C2<double>*)in, (C2<double>*)out, rp, nt, la);
-        else hipLaunchKernelGGL(k_qdht_f64<false>, grid, dim3(256), 0, st, (const double*)amat, (const C2<double>*)in, (C2<double>*)out, rp, nt, la);
+        const int mt = rp / 64, nct = cols ? qc.nct : nt / 32;
+        const dim3 grid(mt * nct + zblocks(mt));
+        if (ls) hipLaunchKernelGGL(k_qdht_f64<true>, grid, dim3(256), 0, st, (const double*)amat, (const C2<double>*)in, (C2<double>*)out, rp, nt, la, qc);
+        else hipLaunchKernelGGL(k_qdht_f64<false>, grid, dim3(256), 0, st, (const double*)amat, (const C2<double>*)in, (C2<double>*)out, rp, nt, la, qc);
     } else if (p == kFP32) {
         if (rp % 128 || nt % 64) return -1001;
-        const dim3 grid((rp / 128) * (nt / 64));
-        if (ls) hipLaunchKernelGGL(k_qdht_f32<true>, grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
-        else hipLaunchKernelGGL(k_qdht_f32<false>, grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
+        const int mt = rp / 128;
+        if (cols) {
+            const dim3 grid(mt * qc.nct + zblocks(mt));
+            if (ls) hipLaunchKernelGGL((k_qdht_f32<true, 32>), grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
+            else hipLaunchKernelGGL((k_qdht_f32<false, 32>), grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
+        } else {
+            const dim3 grid(mt * (nt / 64));
+            if (ls) hipLaunchKernelGGL((k_qdht_f32<true, 64>), grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
+            else hipLaunchKernelGGL((k_qdht_f32<false, 64>), grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
+        }
     } else {
         if (rp % 128 || nt % 128) return -1001;
-        const dim3 grid((rp / 128) * (nt / 128));
-        auto kt = k_qdht_bf16x3_t128<true>;
-        auto kf = k_qdht_bf16x3_t128<false>;
-        HIPCHK(hipFuncSetAttribute((const void*)kt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBf16T128Smem));
-        HIPCHK(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBf16T128Smem));
-        if (ls) hipLaunchKernelGGL(kt, grid, dim3(256), kBf16T128Smem, st, (const uint16_t*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
-        else hipLaunchKernelGGL(kf, grid, dim3(256), kBf16T128Smem, st, (const uint16_t*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
+        const int mt = rp / 128;
+        if (cols) {
+            const dim3 grid(mt * qc.nct + zblocks(mt));
+            if (ls) hipLaunchKernelGGL(k_qdht_bf16x3_t32<true>, grid, dim3(256), 0, st, (const uint16_t*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
+            else hipLaunchKernelGGL(k_qdht_bf16x3_t32<false>, grid, dim3(256), 0, st, (const uint16_t*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
+        } else {
+            const dim3 grid(mt * (nt / 128));
+            auto kt = k_qdht_bf16x3_t128<true>;
+            auto kf = k_qdht_bf16x3_t128<false>;
+            HIPCHK(hipFuncSetAttribute((const void*)kt, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBf16T128Smem));
+            HIPCHK(hipFuncSetAttribute((const void*)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBf16T128Smem));
+            if (ls) hipLaunchKernelGGL(kt, grid, dim3(256), kBf16T128Smem, st, (const uint16_t*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
+            else hipLaunchKernelGGL(kf, grid, dim3(256), kBf16T128Smem, st, (const uint16_t*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la);
+        }
     }
     HIPCHK(hipGetLastError());
     return 0;

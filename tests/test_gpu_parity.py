@@ -455,18 +455,24 @@ def test_mixed_state_keeps_sub_ulp_increments(tmp_path):
 
 
 def test_mixed_linear_phase_drift_without_nonlinearity(tmp_path):
-    """A linear march (NONLIN_REFRINDEX 0, no ionization at 1 nJ, so B1 = 0) at
-    dz = 1e-13 m: each step rotates the phase of every point by dz (-HO - s k1),
-    ~1e-8 rad near the carrier, below half an FP32 ulp.  The inputs of a point
-    then repeat from step to step apart from the state itself, so the
-    unbiased-rounding draw must change with the step (its STEP_N seed), or a
-    point whose first draw keeps its value stays frozen.  Over 100 000 steps the
-    mixed field follows FP64's phase drift to a random walk's error, while the
-    FP32 state rounded to nearest loses most of the drift."""
-    cfg = write_cfg(tmp_path / "c.txt", **dict(HOT, N_T="256", ENERGY="1e-9", NONLIN_REFRINDEX="0"))
+    """A linear march at dz = 1e-13 m: NONLIN_REFRINDEX 0 (the pulse set by POWER,
+    since 0 makes the reference's critical power, and so an ENERGY-specified
+    field, NaN: pulse_gg.cpp:35-36) and no ionization (3 kW), so B1 = 0 exactly.
+    Each step rotates every point by dz (-HO - s k1), ~1.1e-8 rad near the
+    carrier, and the field is turned by pi/4 first so both components of the
+    large points are comparable: the rotation then moves each by under half an
+    FP32 ulp.  A point's inputs then repeat from step to step apart from the
+    state itself, so the unbiased-rounding draw must change with the step (its
+    STEP_N seed), or a point whose first draw keeps its value stays frozen.  Over
+    100 000 steps the mixed field follows FP64's phase drift to a random walk's
+    error, while the FP32 state, rounded to nearest, loses it."""
+    cfg = write_cfg(tmp_path / "c.txt", **dict(HOT, N_T="256", POWER="3e3", NONLIN_REFRINDEX="0"))
+    turn = np.exp(0.25j * np.pi)
     out = {}
     for prec in ("fp64", "mixed", "fp32"):
         p = gsa.Plan(cfg, precision=prec)
+        p.put(p.get() * turn)
+        p.put(p.get(gsa.B2) * turn, gsa.B2)
         f0 = p.get()
         p.fixed_steps(1e-13, 100000, timings=False)
         out[prec] = p.get()
@@ -474,6 +480,42 @@ def test_mixed_linear_phase_drift_without_nonlinearity(tmp_path):
     drift = rel_l2(out["fp64"], f0)
     e_mix, e_32 = rel_l2(out["mixed"], out["fp64"]), rel_l2(out["fp32"], out["fp64"])
     print("linear phase drift %.2e: mixed error %.2e, fp32 error %.2e" % (drift, e_mix, e_32))
-    assert drift > 1e-4
+    assert drift > 5e-4
     assert e_mix < 0.05 * drift
-    assert e_32 > 0.3 * drift
+    assert e_32 > 0.5 * drift
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+def test_column_skipping_equals_full_transforms(tmp_path, prec):
+    """The march's two QDHTs compute only the 32-column blocks that hold a point
+    the linear step keeps (propagator.cu:601-603 zeroes the others): 296 of the
+    512 columns here.  Against the same march with every column transformed
+    (QDHT_SKIP_COLUMNS 0), three fixed steps and then an adaptive march leave
+    FIELD and BIGBUFFER2 identical bit for bit, the skipped columns of FIELD
+    cleared by the first QDHT back in both the host-driven and the
+    device-driven step."""
+    cfg = write_cfg(tmp_path / "c.txt", **HOT)
+    out = []
+    for skip in (0, 1):
+        p = gsa.Plan(cfg, precision=prec)
+        p.set_scalar("QDHT_SKIP_COLUMNS", skip)
+        assert p.scalar("QDHT_SKIP_COLUMNS") == skip
+        nb = p.scalar("QDHT_LIVE_BLOCKS")
+        f0 = p.get()
+        p.fixed_steps(1e-6, 3, timings=False)
+        f3, b3 = p.get(), p.get(gsa.B2)
+        p.put(f0)                      # FIELD set again: its skipped columns are non-zero
+        p.march(str(tmp_path / ("m%d" % skip)))
+        out.append((nb, f0, f3, b3, p.get(), p.get(gsa.B2), p.scalar("STEP_N")))
+        p.close()
+    (nb, f0, *full), (_, _, *skip) = out
+    assert 0 < nb < 512 // 32
+    w = gsa.Plan(cfg, precision=prec)
+    dead = np.abs(f0).max(axis=0) > 0
+    omega = w.table("omega")
+    dead &= (omega < w.scalar("OMEGA_MIN")) | (omega > w.scalar("OMEGA_MAX"))
+    w.close()
+    assert dead.sum() > 0
+    for a, b in zip(full, skip):
+        assert np.array_equal(a, b)
+    assert np.all(full[0][:, dead] == 0) and np.all(full[2][:, dead] == 0)

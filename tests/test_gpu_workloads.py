@@ -85,7 +85,7 @@ def c0(tmp_path_factory):
     o = oracle.Sim(cfg, nthreads=NTH)
     o.run(od)
     out = dict(cfg=cfg, dir=d, steps=o.scalar("STEP_N"), rejects=o.scalar("n_reject"), z=o.scalar("CURRENT_Z"),
-               field=o.get("field"), files={k: oracle.load_udf(os.path.join(od, "c0_%s.udf" % k))
+               field=o.get("field"), files={k: oracle.load_udf(os.path.join(od, "c0_%s.udf" % k))[1]
                                             for k in ("f", "pm", "fa")})
     o.close()
     return out
@@ -107,7 +107,7 @@ def test_c0_adaptive_march_vs_oracle(c0, prec, tmp_path):
     print("c0 march", prec, "steps", c0["steps"], "field", ef)
     assert ef < tol
     for k in ("f", "pm", "fa"):
-        got, ref = oracle.load_udf(os.path.join(out, "c0_%s.udf" % k)), c0["files"][k]
+        got, ref = oracle.load_udf(os.path.join(out, "c0_%s.udf" % k))[1], c0["files"][k]
         assert got.shape == ref.shape
         for z in range(ref.shape[0]):
             e = rel_l2(got[z], ref[z])
