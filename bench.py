@@ -121,7 +121,14 @@ def accuracy_check(gsa, workloads, work, z_max, planes, precisions=("fp64", "mix
     for prec in precisions:
         od = os.path.join(d, prec)
         os.makedirs(od, exist_ok=True)
-        p = gsa.Plan(cfg, precision="fp64" if prec == "fp64_perturbed" else prec)
+        # "mixed:x6back" etc.: the mixed policy with GS_MIX_QDHT set for this run
+        base, _, variant = prec.partition(":")
+        if variant:
+            os.environ["GS_MIX_QDHT"] = variant
+        try:
+            p = gsa.Plan(cfg, precision="fp64" if base == "fp64_perturbed" else base)
+        finally:
+            os.environ.pop("GS_MIX_QDHT", None) if variant else None
         if prec == "fp64_perturbed":
             # sensitivity probe: the FP64 march from FIELD and BIGBUFFER2 with seeded
             # relative noise of size `perturb` (the march's state at z = 0)
@@ -143,13 +150,16 @@ def accuracy_check(gsa, workloads, work, z_max, planes, precisions=("fp64", "mix
         out[prec] = {"field": p.get(), "flux": read_udf(os.path.join(od, "acc_f.udf")),
                      "pm": read_udf(os.path.join(od, "acc_pm.udf")), "steps": int(p.scalar("STEP_N")),
                      "rejected": int(p.scalar("n_reject")), "wall_s": wall, "z": znet, "z_end": z_end,
-                     "reached": sum(1 for k in range(1, planes + 1) if z_end >= znet[k])}
+                     "reached": sum(1 for k in range(1, planes + 1) if z_end >= znet[k]),
+                     "scan_fallbacks": int(p.scalar("SCAN_FALLBACKS"))}
         p.close()
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b)) if np.linalg.norm(b) > 0 else float(np.abs(a).max())
     first = out[precisions[0]]
     res = {"z_planes_m": first["z"][1:], "steps": {k: v["steps"] for k, v in out.items()},
            "rejected": {k: v["rejected"] for k, v in out.items()},
-           "adaptive_steps_per_s": {k: v["steps"] / v["wall_s"] for k, v in out.items()}}
+           "adaptive_steps_per_s": {k: v["steps"] / v["wall_s"] for k, v in out.items()},
+           # line launches whose plasma scan fell back to the chunk-sequential pass
+           "plasma_scan_fallbacks": {k: v["scan_fallbacks"] for k, v in out.items()}}
     if max_steps > 0:
         res["max_steps"] = max_steps
         res["z_reached_m"] = {k: v["z_end"] for k, v in out.items()}
@@ -157,7 +167,7 @@ def accuracy_check(gsa, workloads, work, z_max, planes, precisions=("fp64", "mix
         return res
     ref = out["fp64"]
     common = min(v["reached"] for v in out.values())
-    for prec in (q for q in ("mixed", "fp32", "fp64_perturbed") if q in out):
+    for prec in (q for q in out if q != "fp64"):
         o = out[prec]
         res[prec] = {
             "field_rel_l2_last_plane": rel(o["field"], ref["field"]) if common == planes else None,
@@ -274,7 +284,8 @@ def plane_march(gsa, workloads, work, prec, z_plane, state_dir, resume, time_lim
     stopping early once `time_limit` seconds have passed.  The state (FIELD and
     FIELD_k on the 32-column blocks the march transforms -- every other column
     of both is zero -- CURRENT_Z, ZSTEP and the step counts) is written to
-    state_dir/<prec>_state.npz, and `resume` continues from such a file, so a
+    state_dir/<prec>_state.npz, and `resume` (None or the path of such a file)
+    continues from it, so a
     march longer than one session runs in pieces and ends where an
     uninterrupted one would (FP64 exactly).  At the plane, the per-line
     fluence TSTEP sum_t |E|^2, peak intensity and peak plasma density
@@ -288,7 +299,7 @@ def plane_march(gsa, workloads, work, prec, z_plane, state_dir, resume, time_lim
     path = os.path.join(state_dir, "%s_state.npz" % prec)
     steps0 = rej0 = 0
     if resume:
-        st = np.load(path)
+        st = np.load(resume)
         for buf, key in ((gsa.FIELD, "field"), (gsa.FK, "fk")):
             a = np.zeros((p.R, p.T), complex)
             a[:, st["cols"]] = st[key]
@@ -333,7 +344,11 @@ def gradient_check(gsa, workloads, work, z_max, ndir, energy, max_steps, objecti
     (PhaseMarch: gs_adjoint_forward / _backward), on the step sizes the
     reference controller chose (recorded once in FP64; at most max_steps).
     Reports the time of one gradient (forward with checkpoints + backward; the
-    second of two, after the checkpoint memory is allocated) per precision
+    second of two, after the checkpoint memory is allocated) per precision,
+    both with the default checkpoint budget (every step's FIELD kept when it
+    fits 60 % of the free device memory) and with segments of ~sqrt(2 n) steps
+    forced (ADJOINT_CHECKPOINT_BYTES = 1: one snapshot per segment, the
+    segment recomputed in the backward; the gradient must be bit-identical),
     and, along ndir seeded random phase directions, the relative
     mismatch of each gradient's directional derivative against central
     differences of the FP64 forward, and the relative L2 distance of the
@@ -367,35 +382,57 @@ def gradient_check(gsa, workloads, work, z_max, ndir, energy, max_steps, objecti
             return (torch.fft.ifft(field[0]).abs() ** 4).sum()
         desc = "sum_t |E(0,t)|^4"
 
-    out = {"workload": "N_r=512 x N_t=2048, 0.5 mm, 400 fs, fused silica, %g uJ 50 fs w0=60 um, z %g m"
-                       % (energy * 1e6, z_end),
+    out = {"workload": "N_r=512 x N_t=2048, 0.5 mm, 400 fs, fused silica, %g uJ 50 fs w0=60 um, z %g m, "
+                       "objective %s%s" % (energy * 1e6, z_end, desc,
+                                           "" if (energy == 1e-6 and objective_name == "band") else
+                                           " (substituted for the 1 uJ pulse and the on-axis 500-600 nm band "
+                                           "energy: 1 uJ collapses near 4 mm and needs ~7e5 steps to 1 cm, "
+                                           "and below collapse the band holds ~1e-23 of the on-axis energy)"),
            "params": int(field0.shape[1]), "steps": n, "z_m": z_end, "objective": desc,
            "directions": ndir}
     grads, plans = {}, {}
+    def one_gradient(p):
+        phi = torch.zeros(p.T, dtype=torch.float64, requires_grad=True)
+        t0 = time.perf_counter()
+        L = objective(PhaseMarch.apply(phi, p, field0, dz_rk, dz_lin))
+        t1 = time.perf_counter()
+        L.backward()
+        t2 = time.perf_counter()
+        return phi.grad.numpy().copy(), t1 - t0, t2 - t1
+
     for prec in ("mixed", "fp32", "fp64"):
         p = gsa.Plan(cfg, precision=prec)
         plans[prec] = p
+        fb = p.R * p.T * (16 if prec == "fp64" else 8)
         # the first gradient allocates the checkpoints (up to 150 GiB); the second is timed
-        for rep in range(2):
-            phi = torch.zeros(p.T, dtype=torch.float64, requires_grad=True)
-            t0 = time.perf_counter()
-            L = objective(PhaseMarch.apply(phi, p, field0, dz_rk, dz_lin))
-            t1 = time.perf_counter()
-            L.backward()
-            t2 = time.perf_counter()
-            if rep == 0:
-                print("[bench] gradient %s: first call %.3f s (checkpoint allocation included)" % (prec, t2 - t0),
-                      file=sys.stderr, flush=True)
-        grads[prec] = phi.grad.numpy().copy()
+        _, tf, tb = one_gradient(p)
+        print("[bench] gradient %s: first call %.3f s (checkpoint allocation included)" % (prec, tf + tb),
+              file=sys.stderr, flush=True)
+        g, tf, tb = one_gradient(p)
+        grads[prec] = g
+        K = int(p.scalar("ADJOINT_SEGMENT"))
+        out[prec] = {"forward_s": tf, "backward_s": tb, "gradient_s": tf + tb, "backward_over_forward": tb / tf,
+                     "segment_steps": K,
+                     "checkpoint_gib": (n if K >= n else 2 * -(-n // K) + K + 3) * fb / 2**30}
+        # the same gradient with sqrt(2 n)-step segments forced (one recomputation)
+        p.set_scalar("ADJOINT_CHECKPOINT_BYTES", 1.0)
+        one_gradient(p)
+        gs_, tf2, tb2 = one_gradient(p)
+        K2 = int(p.scalar("ADJOINT_SEGMENT"))
+        p.set_scalar("ADJOINT_CHECKPOINT_BYTES", -1.0)
+        out[prec]["sqrt_checkpointing"] = {
+            "segment_steps": K2, "checkpoint_gib": (2 * -(-n // K2) + K2 + 3) * fb / 2**30,
+            "forward_s": tf2, "backward_s": tb2, "gradient_s": tf2 + tb2,
+            "same_gradient_as_default": bool(np.array_equal(gs_, g))}
         if prec == "fp64" and objective_name == "band":
             with torch.no_grad():
                 f = PhaseMarch.apply(torch.zeros(p.T, dtype=torch.float64), p, field0, dz_rk, dz_lin)
                 out["band_energy_fraction_on_axis"] = float(objective(f) / (f[0].abs() ** 2).sum())
-        out[prec] = {"forward_s": t1 - t0, "backward_s": t2 - t1, "gradient_s": t2 - t0,
-                     "backward_over_forward": (t2 - t1) / (t1 - t0)}
         if prec != "fp64":
             p.close()      # frees its checkpoints (steps x N_r x N_t complex)
-        print("[bench] gradient %s: forward %.3f s, backward %.3f s" % (prec, t1 - t0, t2 - t1),
+        print("[bench] gradient %s: forward %.3f s, backward %.3f s (segments of %d steps); "
+              "sqrt checkpointing (segments of %d steps): forward %.3f s, backward %.3f s, same gradient %s"
+              % (prec, tf, tb, K, K2, tf2, tb2, out[prec]["sqrt_checkpointing"]["same_gradient_as_default"]),
               file=sys.stderr, flush=True)
     rng = np.random.default_rng(seed)
     p64 = plans["fp64"]
@@ -485,10 +522,12 @@ def main():
                          "and, at the plane, its field and line diagnostics under --state-dir")
     ap.add_argument("--plane-z", type=float, default=1e-2)
     ap.add_argument("--state-dir", default="gpurun_out/plane")
-    ap.add_argument("--resume", action="store_true", help="continue the --plane-march from its saved state")
+    ap.add_argument("--resume", default=None, metavar="STATE_NPZ",
+                    help="continue the --plane-march from a state file an earlier --plane-march wrote")
     ap.add_argument("--time-limit", type=float, default=900.0, help="seconds of --plane-march before it saves")
     ap.add_argument("--fft-modes", action="store_true",
-                    help="only compare the mixed march with its line FFTs on VALU or on MFMA (GS_MIX_FFT modes)")
+                    help="only compare the mixed march with its line FFTs on VALU or on MFMA (GS_MIX_FFT modes) "
+                         "and its QDHTs with a two- or three-part bf16 split (GS_MIX_QDHT modes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device-to-device copy (reference kernel for the HBM byte counters)")
     args = ap.parse_args()
@@ -562,13 +601,15 @@ def main():
     if args.fft_modes:
         ref = run_precision(gsa, cfg, "fp64", args.steps, args.warmup, dz)["field"]
         out = {}
-        for mode in ("valu", "kerr", "fwd", "kerr_inv", "all"):
-            os.environ["GS_MIX_FFT"] = mode
+        for var, mode in [("GS_MIX_FFT", m) for m in ("valu", "kerr", "fwd", "kerr_inv", "all")] + \
+                         [("GS_MIX_QDHT", m) for m in ("x6back", "x6")]:
+            os.environ[var] = mode
             r = run_precision(gsa, cfg, "mixed", args.steps, args.warmup, dz)
-            out[mode] = {"steps_per_s": r["steps_per_s"], "kernel_ms": r["kernel_ms"],
-                         "field_rel_l2_vs_fp64": float(np.linalg.norm(r["field"] - ref) / np.linalg.norm(ref))}
-            print("[bench] GS_MIX_FFT=%s: %s" % (mode, out[mode]), file=sys.stderr, flush=True)
-        os.environ.pop("GS_MIX_FFT")
+            os.environ.pop(var)
+            out["%s=%s" % (var, mode)] = {"steps_per_s": r["steps_per_s"], "kernel_ms": r["kernel_ms"],
+                                          "field_rel_l2_vs_fp64": float(np.linalg.norm(r["field"] - ref)
+                                                                        / np.linalg.norm(ref))}
+            print("[bench] %s=%s: %s" % (var, mode, out["%s=%s" % (var, mode)]), file=sys.stderr, flush=True)
         print(json.dumps({"fft_modes": out}))
         return
 
@@ -597,7 +638,7 @@ def main():
     cols = m["qdht_columns"]
     qdht_ms = 0.5 * (km["qdht_lstep"] + km["qdht_back"])
     dominant = "nl" if km["nl_rk4"] >= km["qdht_lstep"] + km["qdht_back"] else "qdht"
-    roof = (roofline("qdht", qdht_ms, R, T, "mixed", "k_qdht_bf16x3_t32<false>", cols) if dominant == "qdht"
+    roof = (roofline("qdht", qdht_ms, R, T, "mixed", "k_qdht_bf16x6_t32<false>", cols) if dominant == "qdht"
             else roofline("nl", km["nl_rk4"], R, T, "mixed", "k_nl_rk4<PolMix>"))
 
     acc = None
@@ -626,8 +667,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "mixed: bf16x3-mfma/f32-acc QDHT and forward line FFTs, f32 inverse FFT, f64 plasma scan, "
-                 "linear-step update and norms",
+        "dtype": "mixed: split-bf16 mfma/f32-acc QDHTs (two-part split forward, three-part back), f32 line FFTs, "
+                 "f64 plasma scan, linear-step update and norms",
         "data": "synthetic (config-built analytic pulse, seeded 1% amplitude noise)",
         "config": {"workload": WORKLOAD,
                    "grid": [R, T], "dz_m": dz, "mode": "fixed-dz", "precision": "mixed"},
@@ -648,12 +689,12 @@ def main():
         "phase_gradient": grad,
         "kernel_ms_per_step": {p: res[p]["kernel_ms"] for p in res},
         "roofline": roof,
-        "roofline_qdht": roofline("qdht", km["qdht_back"], R, T, "mixed", "k_qdht_bf16x3_t32<false>", cols),
+        "roofline_qdht": roofline("qdht", km["qdht_back"], R, T, "mixed", "k_qdht_bf16x6_t32<false>", cols),
         "roofline_qdht_lstep": roofline("qdht", km["qdht_lstep"], R, T, "mixed", "k_qdht_bf16x3_t32<true>", cols),
         "roofline_qdht_fp32": roofline("qdht", res["fp32"]["kernel_ms"]["qdht_back"], R, T, "fp32",
                                        "k_qdht_f32<false, 32>", res["fp32"]["qdht_columns"]),
-        "roofline_qdht_all_columns": roofline("qdht", full["mixed"]["kernel_ms"]["qdht_back"], R, T, "mixed",
-                                              "k_qdht_bf16x3_t128<false>"),
+        "roofline_qdht_all_columns": roofline("qdht", full["mixed"]["kernel_ms"]["qdht_lstep"], R, T, "mixed",
+                                              "k_qdht_bf16x3_t128<true>"),
         "traffic_profile_current": traffic_profile_current(),
         "roofline_nl": roofline("nl", km["nl_rk4"], R, T, "mixed", "k_nl_rk4<PolMix>"),
         "roofline_nl_valu": valu_roofline("k_nl_rk4<PolMix>", km["nl_rk4"]),

@@ -58,6 +58,7 @@ struct QCols {
     const int* ztile;   // first column of each skipped 32-column block
     int nzt;
     int zero_dead;      // host-driven launches: clear the skipped blocks of the output
+    int split3;         // mixed: operands in three bf16 parts, six products (FP32-level accuracy)
 };
 
 // Scalars and table pointers the NL kernels read.  Table pointers refer to

@@ -72,6 +72,19 @@ template <int M, int E> struct PickRadix {
     static constexpr int v = (E >= 16 && M % 16 == 0) ? 16 : (E >= 8 && M % 8 == 0) ? 8 : (M % 4 == 0) ? 4 : 2;
 };
 
+// Twiddles of a pass of span L = Ns RADIX: v[r] *= w_L^{r k}, w_L = e^{-2 pi i/L}
+// (conjugated for SIGN > 0), from the one table entry w_L^k = tw[k T/L] by the
+// product tree of twiddle_pow.  (A precomputed per-pass table in lane order,
+// one 64-byte load per butterfly instead of six complex products, measured
+// slower: the line kernels run two waves per SIMD and do not hide the loads.)
+template <class R, int T, int RADIX, int SIGN>
+__device__ __forceinline__ void twiddle_apply(C2<R>* v, const C2<R>* __restrict__ tw, int L, int k)
+{
+    C2<R> w = tw[k * (T / L)];
+    if (SIGN > 0) w.y = -w.y;
+    twiddle_pow<RADIX>(v, w);
+}
+
 // One Stockham pass src -> dst (both padded LDS buffers), no barrier.
 template <class R, int T, int NT, int E, int RADIX, int SIGN>
 __device__ __forceinline__ void spass(const C2<R>* __restrict__ src, C2<R>* __restrict__ dst,
@@ -87,11 +100,7 @@ __device__ __forceinline__ void spass(const C2<R>* __restrict__ src, C2<R>* __re
 #pragma unroll
             for (int r = 0; r < RADIX; ++r) v[r] = src[pidx(j + r * NB, E)];
             const int k = j & (Ns - 1);
-            if (Ns > 1) {
-                C2<R> w = tw[k * (T / (Ns * RADIX))];
-                if (SIGN > 0) w.y = -w.y;
-                twiddle_pow<RADIX>(v, w);
-            }
+            if (Ns > 1) twiddle_apply<R, T, RADIX, SIGN>(v, tw, Ns * RADIX, k);
             dft_small<RADIX, SIGN>(v);
             const int d = (j - k) * RADIX + k;
 #pragma unroll
@@ -138,7 +147,7 @@ __device__ __forceinline__ void fft_fwd_acc(const C2<R> (&x)[E], const F (&fac)[
     C2<R> y[E];
 #pragma unroll
     for (int r = 0; r < E; ++r) y[r] = src[pidx(tid + r * NT, E)];
-    twiddle_pow<E>(y, tw[tid]);
+    twiddle_apply<R, T, E, -1>(y, tw, T, tid);
     dft_small<E, -1>(y);
 #pragma unroll
     for (int r = 0; r < E; ++r) {

@@ -116,10 +116,19 @@ struct gs_plan {
     int nct = 0, nzt = 0;
     bool qskip = true;
     bool dead_dirty = true;
-    gs::QCols qcols(bool zero_dead) const
+    // mixed: which of the march's transforms split their operands in three bf16 parts
+    // (GS_MIX_QDHT x6back, the default: the QDHT back; x6: both; x3: neither)
+    bool split3_fwd = false, split3_back = false;
+    // the column selection of one of the march's QDHTs: the live blocks, or every
+    // column when skipping is off (through the narrow tiles when split3 asks for them;
+    // nullptr: the plain full-width kernels)
+    gs::QCols qcols(bool zero_dead, bool back) const
     {
-        return gs::QCols{ctile.as<int>(), nct, ztile.as<int>(), nzt, zero_dead ? 1 : 0};
+        const int s3 = (back ? split3_back : split3_fwd) ? 1 : 0;
+        if (!qskip) return gs::QCols{nullptr, 0, nullptr, 0, 0, s3};
+        return gs::QCols{ctile.as<int>(), nct, ztile.as<int>(), nzt, zero_dead ? 1 : 0, s3};
     }
+    bool march_cols(bool back) const { return qskip || (back ? split3_back : split3_fwd); }
 
     double ZSTEP = 0, CURRENT_Z = 0;
     long STEP_N = 0, n_reject = 0;
@@ -216,19 +225,25 @@ void download_field64(gs_plan* p, const void* src, double* host, size_t nlines)
 // lo = bf16(a - hi), in the MFMA fragment order of the GEMM: [m/128][k/32][k slice
 // (k%32)/16][32-row block][hi, lo][lane][8], lane = m%32 + 32 ((k%16)/8), element
 // k%8 (v_mfma_f32_32x32x16_bf16 A operand)
+// A third plane of the same fragment order without the hi/lo interleave, at
+// 2 rp^2, holds the rest bf16(a - hi - lo) for the three-part split of the
+// column-skipping GEMM.
 void upload_bf16_fragments(DevBuf& dst, const std::vector<double>& M, int rp)
 {
-    std::vector<uint16_t> fr((size_t)2 * rp * rp);
+    std::vector<uint16_t> fr((size_t)3 * rp * rp);
     const int nk = rp / 32;
     for (int m = 0; m < rp; ++m)
         for (int k = 0; k < rp; ++k) {
-            const float a = (float)M[(size_t)m * rp + k];
+            const double ad = M[(size_t)m * rp + k];
+            const float a = (float)ad;
             const uint16_t h = bf16_rn_host(a), l = bf16_rn_host(a - bf16_to_float(h));
+            const uint16_t r = bf16_rn_host((float)(ad - bf16_to_float(h) - bf16_to_float(l)));
             const int mt = m / 128, rbb = (m % 128) / 32, step = k / 32, kk = (k % 32) / 16;
             const int lane = m % 32 + 32 * ((k % 16) / 8), e = k % 8;
             const size_t base = (((((size_t)mt * nk + step) * 2 + kk) * 4 + rbb) * 2) * 64 * 8;
             fr[base + lane * 8 + e] = h;
             fr[base + 64 * 8 + lane * 8 + e] = l;
+            fr[(size_t)2 * rp * rp + base / 2 + lane * 8 + e] = r;
         }
     upload(dst, fr.data(), fr.size() * 2);
 }
@@ -362,6 +377,14 @@ void build_plan(gs_plan* p)
         if (p->nct > 0) upload(p->ctile, ct.data(), ct.size() * sizeof(int));
         if (p->nzt > 0) upload(p->ztile, zt.data(), zt.size() * sizeof(int));
         p->qskip = p->nct > 0 && !std::getenv("GS_QDHT_ALL_COLUMNS");
+        if (p->prec == gs::kMIXED) {
+            const char* m = std::getenv("GS_MIX_QDHT");
+            const std::string mode = m ? m : "x6back";
+            if (mode != "x3" && mode != "x6back" && mode != "x6")
+                throw gs::Error(GS_E_ARG, "GS_MIX_QDHT must be x3, x6back or x6");
+            p->split3_back = mode != "x3";
+            p->split3_fwd = mode == "x6";
+        }
         if (p->prec == gs::kFP32) {
             std::vector<float> rf(rate.begin(), rate.end());
             upload(p->lrate, rf.data(), rf.size() * 4);
@@ -576,11 +599,12 @@ void nl_step(gs_plan* p, double dz, double* maxA2, double* maxNL2)
 void enqueue_linear(gs_plan* p, const gs::LstepArgs& ls, hipEvent_t mid)
 {
     const void* A = p->prec == gs::kFP64 ? p->at64.p : p->amat.p;
-    const gs::QCols fwd = p->qcols(false), back = p->qcols(p->dead_dirty);
-    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->b1.p, nullptr, &ls, p->st, nullptr, p->qskip ? &fwd : nullptr));
+    const gs::QCols fwd = p->qcols(false, false), back = p->qcols(p->dead_dirty, true);
+    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->b1.p, nullptr, &ls, p->st, nullptr,
+                        p->march_cols(false) ? &fwd : nullptr));
     if (mid) HCK(hipEventRecord(mid, p->st));
     LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->fk.p, p->field.p, nullptr, p->st, nullptr,
-                        p->qskip ? &back : nullptr));
+                        p->march_cols(true) ? &back : nullptr));
     p->dead_dirty = false;
 }
 
@@ -648,9 +672,9 @@ void enqueue_slot(gs_plan* p)
     l.ctl = c;
     const void* A = p->prec == gs::kFP64 ? p->at64.p : p->amat.p;
     // the QDHT back carries its clearing blocks; they act only when DevCtl::dead_dirty is set
-    const gs::QCols fwd = p->qcols(false), back = p->qcols(false);
-    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->b1.p, nullptr, &l, p->st, nullptr, p->qskip ? &fwd : nullptr));
-    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->fk.p, p->field.p, nullptr, p->st, c, p->qskip ? &back : nullptr));
+    const gs::QCols fwd = p->qcols(false, false), back = p->qcols(false, true);
+    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->b1.p, nullptr, &l, p->st, nullptr, p->march_cols(false) ? &fwd : nullptr));
+    LCK(gs::launch_qdht(p->prec, p->rp, p->s.NT, A, p->fk.p, p->field.p, nullptr, p->st, c, p->march_cols(true) ? &back : nullptr));
     LCK(gs::launch_finish(c, p->st));
 }
 
@@ -951,8 +975,8 @@ struct BatchMarch {
                     g.NC = p->s.NT;
                     if (lstep) g.la = p->ls;
                     g.la.ctl = dctl + i;
-                    if (p->qskip) g.qc = p->qcols(false);
-                    g.nb = gs::qdht_blocks(pr, p->rp, p->s.NT, p->qskip ? &g.qc : nullptr, !lstep, true);
+                    g.qc = p->qcols(false, !lstep);
+                    g.nb = gs::qdht_blocks(pr, p->rp, p->s.NT, &g.qc, !lstep, true);
                     g.b0 = (int)brun.size();
                     const int padded = (g.nb + 7) / 8 * 8;
                     brun.insert(brun.end(), padded, (int)runs.size());

@@ -383,7 +383,8 @@ constexpr size_t kBf16T128Smem = t128::SMEM;
 namespace t32 {
 constexpr int BM = 128, BNC = 32, BK = 32, LD = BK + 8;
 constexpr int PLANE = BNC * LD;         // one [32 columns][LD] plane
-constexpr int BUF = 4 * PLANE;          // B hi re, B hi im, B lo re, B lo im
+// a buffer holds NS bf16 parts of the K step's B, re and im: plane 2 part + (0 re, 1 im)
+template <int NS> constexpr int buf_elems() { return 2 * NS * PLANE; }
 // global loads of one K step: 4 consecutive k rows of one column
 __device__ __forceinline__ void gload_b(const C2<float>* gX, int NC, int step, int nk, float (&xr)[4], float (&xi)[4])
 {
@@ -396,36 +397,58 @@ __device__ __forceinline__ void gload_b(const C2<float>* gX, int NC, int step, i
         xi[j] = v.y;
     }
 }
-// the wave's A fragments of one K step: [kk][hi/lo] of its 32-row block
-__device__ __forceinline__ void gload_a(const uint16_t* A, int mi, int nk, int step, int rbb, int lane, v8bf (&a)[2][2])
+// the wave's A fragments of one K step: [kk][part] of its 32-row block; parts 0, 1
+// (hi, lo) interleaved as in t128, part 2 (the remainder) in a plane of its own
+// behind them (A + 2 K^2)
+template <int NS>
+__device__ __forceinline__ void gload_a(const uint16_t* A, int K, int mi, int nk, int step, int rbb, int lane,
+                                        v8bf (&a)[2][NS])
 {
     step = step < nk ? step : nk - 1;
 #pragma unroll
-    for (int kk = 0; kk < 2; ++kk)
+    for (int kk = 0; kk < 2; ++kk) {
 #pragma unroll
         for (int hl = 0; hl < 2; ++hl) a[kk][hl] = *(const v8bf*)(A + t128::afrag(mi, nk, step, kk, rbb, hl) + lane * 8);
+        if constexpr (NS == 3)
+            a[kk][2] = *(const v8bf*)(A + (size_t)2 * K * K + t128::afrag(mi, nk, step, kk, rbb, 0) / 2 + lane * 8);
+    }
 }
+// the K step's B split into NS bf16 parts (x = hi + lo (+ rest), each the
+// nearest bf16 of what the earlier parts leave) into LDS buffer b
+template <int NS>
 __device__ __forceinline__ void sstore_b(uint16_t* b, int oB, const float (&xr)[4], const float (&xi)[4])
 {
-    uint32_t hr[2], lr[2], hi[2], li[2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        split2(xr[2 * j], xr[2 * j + 1], &hr[j], &lr[j]);
-        split2(xi[2 * j], xi[2 * j + 1], &hi[j], &li[j]);
-    }
     typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-    *(u32x2*)(b + oB) = u32x2{hr[0], hr[1]};
-    *(u32x2*)(b + oB + PLANE) = u32x2{hi[0], hi[1]};
-    *(u32x2*)(b + oB + 2 * PLANE) = u32x2{lr[0], lr[1]};
-    *(u32x2*)(b + oB + 3 * PLANE) = u32x2{li[0], li[1]};
+#pragma unroll
+    for (int pt = 0; pt < 2; ++pt) {
+        const float(&x)[4] = pt ? xi : xr;
+        uint32_t h[2], l[2], r2[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            split2(x[2 * j], x[2 * j + 1], &h[j], &l[j]);
+            if constexpr (NS == 3) {
+                const f32x2_t v = {x[2 * j], x[2 * j + 1]};
+                const f32x2_t rem = v - __builtin_convertvector(__builtin_bit_cast(bf16x2_t, h[j]), f32x2_t) -
+                                    __builtin_convertvector(__builtin_bit_cast(bf16x2_t, l[j]), f32x2_t);
+                r2[j] = __builtin_bit_cast(uint32_t, __builtin_convertvector(rem, bf16x2_t));
+            }
+        }
+        *(u32x2*)(b + oB + pt * PLANE) = u32x2{h[0], h[1]};
+        *(u32x2*)(b + oB + (2 + pt) * PLANE) = u32x2{l[0], l[1]};
+        if constexpr (NS == 3) *(u32x2*)(b + oB + (4 + pt) * PLANE) = u32x2{r2[0], r2[1]};
+    }
 }
 }  // namespace t32
 
-template <bool LSTEP>
+// NS = 2: split-BF16 3-pass (lo*hi, hi*lo, hi*hi); NS = 3: both operands in three
+// bf16 parts (to ~2^-24, the FP32 accuracy of the field), six products
+// (mid*mid, hi*rest, rest*hi, mid*hi, hi*mid, hi*hi, smallest first)
+template <bool LSTEP, int NS = 2>
 __device__ __forceinline__ void qdht_t32_body(int bid, const uint16_t* __restrict__ Afr, const C2<float>* __restrict__ X,
                                               C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, const QCols& qc)
 {
     using namespace t32;
+    constexpr int BUF = buf_elems<NS>();
     __shared__ __align__(16) uint16_t sm[2 * BUF];
     const DevCtl* ctl = la.ctl;
     if (ctl) {
@@ -456,66 +479,72 @@ __device__ __forceinline__ void qdht_t32_body(int bid, const uint16_t* __restric
         for (int r = 0; r < 16; ++r) acc[a][r] = 0.f;
 
     struct OpsB {
-        v8bf h[2][2], l[2][2];   // [kk][pt]
+        v8bf p[NS][2][2];   // [part][kk][pt]
     };
     auto read_b = [&](const uint16_t* b, OpsB& o) {
 #pragma unroll
-        for (int kk = 0; kk < 2; ++kk)
+        for (int q = 0; q < NS; ++q)
 #pragma unroll
-            for (int pt = 0; pt < 2; ++pt) {
-                o.h[kk][pt] = *(const v8bf*)(b + rB + pt * PLANE + 16 * kk);
-                o.l[kk][pt] = *(const v8bf*)(b + rB + (2 + pt) * PLANE + 16 * kk);
-            }
+            for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+                for (int pt = 0; pt < 2; ++pt) o.p[q][kk][pt] = *(const v8bf*)(b + rB + (2 * q + pt) * PLANE + 16 * kk);
     };
-    auto mfma_ops = [&](const v8bf (&a)[2][2], const OpsB& o) {
+    auto mfma_ops = [&](const v8bf (&a)[2][NS], const OpsB& o) {
+        auto mm = [&](int kk, int qa, int qb) {
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt)
+                acc[pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[kk][qa], o.p[qb][kk][pt], acc[pt], 0, 0, 0);
+        };
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
-#pragma unroll
-            for (int pt = 0; pt < 2; ++pt) acc[pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[kk][1], o.h[kk][pt], acc[pt], 0, 0, 0);
-#pragma unroll
-            for (int pt = 0; pt < 2; ++pt) acc[pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[kk][0], o.l[kk][pt], acc[pt], 0, 0, 0);
-#pragma unroll
-            for (int pt = 0; pt < 2; ++pt) acc[pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[kk][0], o.h[kk][pt], acc[pt], 0, 0, 0);
+            if constexpr (NS == 3) {
+                mm(kk, 1, 1);
+                mm(kk, 0, 2);
+                mm(kk, 2, 0);
+            }
+            mm(kk, 1, 0);
+            mm(kk, 0, 1);
+            mm(kk, 0, 0);
         }
     };
-    // interleave the 12 MFMAs of a step with the split VALU, the LDS writes and the global loads
+    // interleave the MFMAs of a step with the split VALU, the LDS writes and the global loads
     auto schedule = [&]() {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
-            __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);   // VALU
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);   // DS write
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x020, 2, 0);   // VMEM read
+            __builtin_amdgcn_sched_group_barrier(0x008, NS - 1, 0);   // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x002, 4 * (NS - 1), 0);   // VALU
+            __builtin_amdgcn_sched_group_barrier(0x008, NS - 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x200, NS - 1, 0);   // DS write
+            __builtin_amdgcn_sched_group_barrier(0x008, NS - 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x020, NS, 0);       // VMEM read
         }
     };
 
     uint16_t* const buf0 = sm;
     uint16_t* const buf1 = sm + BUF;
     float xr[4], xi[4], nxr[4], nxi[4];
-    v8bf a0[2][2], a1[2][2];
+    v8bf a0[2][NS], a1[2][NS];
     gload_b(gX, NC, 0, nk, xr, xi);
-    sstore_b(buf0, oB, xr, xi);
+    sstore_b<NS>(buf0, oB, xr, xi);
     gload_b(gX, NC, 1, nk, xr, xi);
-    gload_a(Afr, mi, nk, 0, w, lane, a0);
+    gload_a<NS>(Afr, K, mi, nk, 0, w, lane, a0);
     __syncthreads();
     OpsB ops;
     for (int k = 0; k < nk; k += 2) {
         read_b(buf0, ops);
         __builtin_amdgcn_sched_barrier(0);
         gload_b(gX, NC, k + 2, nk, nxr, nxi);
-        gload_a(Afr, mi, nk, k + 1, w, lane, a1);
+        gload_a<NS>(Afr, K, mi, nk, k + 1, w, lane, a1);
         mfma_ops(a0, ops);
-        sstore_b(buf1, oB, xr, xi);
+        sstore_b<NS>(buf1, oB, xr, xi);
         schedule();
         __syncthreads();
         read_b(buf1, ops);
         __builtin_amdgcn_sched_barrier(0);
         gload_b(gX, NC, k + 3, nk, xr, xi);
-        gload_a(Afr, mi, nk, k + 2, w, lane, a0);
+        gload_a<NS>(Afr, K, mi, nk, k + 2, w, lane, a0);
         mfma_ops(a1, ops);
-        sstore_b(buf0, oB, nxr, nxi);
+        sstore_b<NS>(buf0, oB, nxr, nxi);
         schedule();
         __syncthreads();
     }
@@ -545,7 +574,13 @@ template <bool LSTEP>
 __global__ __launch_bounds__(256) void k_qdht_bf16x3_t32(const uint16_t* __restrict__ Afr, const C2<float>* __restrict__ X,
                                                          C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, QCols qc)
 {
-    qdht_t32_body<LSTEP>(blockIdx.x, Afr, X, Y, K, NC, la, qc);
+    qdht_t32_body<LSTEP, 2>(blockIdx.x, Afr, X, Y, K, NC, la, qc);
+}
+template <bool LSTEP>
+__global__ __launch_bounds__(256) void k_qdht_bf16x6_t32(const uint16_t* __restrict__ Afr, const C2<float>* __restrict__ X,
+                                                         C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, QCols qc)
+{
+    qdht_t32_body<LSTEP, 3>(blockIdx.x, Afr, X, Y, K, NC, la, qc);
 }
 
 // FP64: tile 64 rows x 32 complex columns, K step 16, 4 waves (2x2)
@@ -725,7 +760,8 @@ __global__ __launch_bounds__(256) void k_qdht_t32_batch(const GemmRun* __restric
     const GemmRun& g = runs[brun[blockIdx.x]];
     const int bid = (int)blockIdx.x - g.b0;
     if (bid >= g.nb) return;
-    qdht_t32_body<LSTEP>(bid, (const uint16_t*)g.A, (const C2<float>*)g.X, (C2<float>*)g.Y, g.K, g.NC, g.la, g.qc);
+    if (g.qc.split3) qdht_t32_body<LSTEP, 3>(bid, (const uint16_t*)g.A, (const C2<float>*)g.X, (C2<float>*)g.Y, g.K, g.NC, g.la, g.qc);
+    else qdht_t32_body<LSTEP, 2>(bid, (const uint16_t*)g.A, (const C2<float>*)g.X, (C2<float>*)g.Y, g.K, g.NC, g.la, g.qc);
 }
 template <bool LSTEP>
 __global__ __launch_bounds__(256) void k_qdht_f32_batch(const GemmRun* __restrict__ runs, const int* __restrict__ brun)
@@ -757,7 +793,7 @@ __global__ void k_finish_batch(DevCtl* ctls, int n)
 int qdht_blocks(Prec p, int rp, int nt, const QCols* cols, bool back, bool gated)
 {
     const int bm = p == kFP64 ? 64 : 128, mt = rp / bm;
-    if (!cols) return mt * (nt / 32);
+    if (!cols || !cols->ctile) return mt * (nt / 32);
     return mt * cols->nct + ((back && (cols->zero_dead || gated)) ? mt * ((cols->nzt + 7) / 8) : 0);
 }
 
@@ -845,22 +881,24 @@ int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* in, void* 
     QCols qc = cols ? *cols : QCols{};
     // blocks of a column-skipping QDHT back that clear the skipped columns: present when the
     // host asks for it, or always in the device-driven march (they read DevCtl::dead_dirty)
+    // cols with no column list: every column, through the narrow tiles (for split3)
     auto zblocks = [&](int mt) {
-        if (!cols || ls || !(qc.zero_dead || gate)) return 0;
+        if (!cols || !qc.ctile || ls || !(qc.zero_dead || gate)) return 0;
         return mt * ((qc.nzt + 7) / 8);
     };
-    if (cols && (nt % 32 || qc.nct < 1)) return -1001;
+    if (cols && (nt % 32 || (qc.ctile && qc.nct < 1))) return -1001;
+    const int ncb = cols && qc.ctile ? qc.nct : nt / 32;     // 32-column blocks computed
     if (p == kFP64) {
         if (rp % 64 || nt % 32) return -1001;
-        const int mt = rp / 64, nct = cols ? qc.nct : nt / 32;
-        const dim3 grid(mt * nct + zblocks(mt));
+        const int mt = rp / 64;
+        const dim3 grid(mt * ncb + zblocks(mt));
         if (ls) hipLaunchKernelGGL(k_qdht_f64<true>, grid, dim3(256), 0, st, (const double*)amat, (const C2<double>*)in, (C2<double>*)out, rp, nt, la, qc);
         else hipLaunchKernelGGL(k_qdht_f64<false>, grid, dim3(256), 0, st, (const double*)amat, (const C2<double>*)in, (C2<double>*)out, rp, nt, la, qc);
     } else if (p == kFP32) {
         if (rp % 128 || nt % 64) return -1001;
         const int mt = rp / 128;
         if (cols) {
-            const dim3 grid(mt * qc.nct + zblocks(mt));
+            const dim3 grid(mt * ncb + zblocks(mt));
             if (ls) hipLaunchKernelGGL((k_qdht_f32<true, 32>), grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
             else hipLaunchKernelGGL((k_qdht_f32<false, 32>), grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
         } else {
@@ -872,9 +910,11 @@ int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* in, void* 
         if (rp % 128 || nt % 128) return -1001;
         const int mt = rp / 128;
         if (cols) {
-            const dim3 grid(mt * qc.nct + zblocks(mt));
-            if (ls) hipLaunchKernelGGL(k_qdht_bf16x3_t32<true>, grid, dim3(256), 0, st, (const uint16_t*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
-            else hipLaunchKernelGGL(k_qdht_bf16x3_t32<false>, grid, dim3(256), 0, st, (const uint16_t*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
+            const dim3 grid(mt * ncb + zblocks(mt));
+            auto k3t = k_qdht_bf16x3_t32<true>, k3f = k_qdht_bf16x3_t32<false>;
+            auto k6t = k_qdht_bf16x6_t32<true>, k6f = k_qdht_bf16x6_t32<false>;
+            auto kf = qc.split3 ? (ls ? k6t : k6f) : (ls ? k3t : k3f);
+            hipLaunchKernelGGL(kf, grid, dim3(256), 0, st, (const uint16_t*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
         } else {
             const dim3 grid(mt * (nt / 128));
             auto kt = k_qdht_bf16x3_t128<true>;

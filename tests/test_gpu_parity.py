@@ -519,3 +519,25 @@ def test_column_skipping_equals_full_transforms(tmp_path, prec):
     for a, b in zip(full, skip):
         assert np.array_equal(a, b)
     assert np.all(full[0][:, dead] == 0) and np.all(full[2][:, dead] == 0)
+
+
+def test_mixed_three_part_split_qdht(tmp_path, monkeypatch):
+    """GS_MIX_QDHT x6: the column-skipping split-BF16 QDHTs take both operands
+    in three bf16 parts (six products): three fixed steps land closer to the
+    FP64 oracle than with the two-part split, within the single-precision
+    tolerance."""
+    cfg = write_cfg(tmp_path / "c.txt", **HOT)
+    o = oracle.Sim(cfg, nthreads=8)
+    for _ in range(3):
+        o.fixed_step(1e-6)
+    ref = o.get("field")
+    err = {}
+    for mode in ("x3", "x6"):
+        monkeypatch.setenv("GS_MIX_QDHT", mode)
+        p = gsa.Plan(cfg, precision="mixed")
+        for _ in range(3):
+            p.fixed_step(1e-6)
+        err[mode] = rel_l2(p.get(), ref)
+        p.close()
+    print("three fixed steps, mixed: x3 %.2e, x6 %.2e" % (err["x3"], err["x6"]))
+    assert err["x6"] < 2e-5 and err["x6"] < err["x3"]
