@@ -41,8 +41,12 @@ def run_precision(gsa, cfg, prec, steps, warmup, dz, skip=True):
     p = gsa.Plan(cfg, precision=prec)
     p.set_scalar("QDHT_SKIP_COLUMNS", 1 if skip else 0)
     live_cols = 32 * p.scalar("QDHT_LIVE_BLOCKS") if skip else p.T
-    print("[bench] %s: line kernel %d samples per thread (autotuned)" % (prec, p.scalar("NL_SAMPLES_PER_THREAD")),
-          file=sys.stderr, flush=True)
+    tiles = {"nl_samples_per_thread": int(p.scalar("NL_SAMPLES_PER_THREAD"))}
+    if prec == "mixed":
+        tiles["qdht_k_groups"] = int(p.scalar("QDHT_K_GROUPS"))
+    elif prec == "fp32" and not skip:
+        tiles["qdht_narrow_tiles"] = bool(p.scalar("QDHT_F32_NARROW_FULL"))
+    print("[bench] %s: autotuned shapes %s" % (prec, tiles), file=sys.stderr, flush=True)
     if warmup > 0:
         p.fixed_steps(dz, warmup, timings=False)
     t0 = time.perf_counter()
@@ -53,7 +57,8 @@ def run_precision(gsa, cfg, prec, steps, warmup, dz, skip=True):
     R, T = p.R, p.T
     p.close()
     return {"wall_s": wall, "steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
-            "kernel_ms": kernel_ms, "field": field, "R": R, "T": T, "qdht_columns": int(live_cols)}
+            "kernel_ms": kernel_ms, "field": field, "R": R, "T": T, "qdht_columns": int(live_cols),
+            "autotuned": tiles}
 
 
 def cpu_baseline(cfg, dz, nsteps, threads):
@@ -203,6 +208,13 @@ def traffic_profile_current():
             return json.load(f).get("kernel_sources_sha16") == kernel_sources_sha()
     except (OSError, ValueError):
         return False
+
+
+def t32_kernel(products, lstep):
+    """rocprofv3 name of the narrow split-BF16 QDHT kernel the mixed march runs
+    (two or three bf16 parts: 3 or 6 products; K groups from GS_QDHT_KSPLIT)."""
+    ks = 1 if os.environ.get("GS_QDHT_KSPLIT") == "1" else 2
+    return "k_qdht_bf16x%d_t32<%s, %d>" % (products, "true" if lstep else "false", ks)
 
 
 def measured_traffic(kernel):
@@ -638,7 +650,7 @@ def main():
     cols = m["qdht_columns"]
     qdht_ms = 0.5 * (km["qdht_lstep"] + km["qdht_back"])
     dominant = "nl" if km["nl_rk4"] >= km["qdht_lstep"] + km["qdht_back"] else "qdht"
-    roof = (roofline("qdht", qdht_ms, R, T, "mixed", "k_qdht_bf16x6_t32<false>", cols) if dominant == "qdht"
+    roof = (roofline("qdht", qdht_ms, R, T, "mixed", t32_kernel(6, False), cols) if dominant == "qdht"
             else roofline("nl", km["nl_rk4"], R, T, "mixed", "k_nl_rk4<PolMix>"))
 
     acc = None
@@ -680,7 +692,9 @@ def main():
         "all_columns": {"mixed": full["mixed"]["steps_per_s"], "fp32": full["fp32"]["steps_per_s"],
                         "mixed_over_fp32": full["mixed"]["steps_per_s"] / full["fp32"]["steps_per_s"],
                         "fields_identical_to_skipping": {k: full[k]["same_field_as_skip"] for k in full},
-                        "kernel_ms_per_step": {k: full[k]["kernel_ms"] for k in full}},
+                        "kernel_ms_per_step": {k: full[k]["kernel_ms"] for k in full},
+                        "autotuned_shapes": {k: full[k]["autotuned"] for k in full}},
+        "autotuned_shapes": {k: res[k]["autotuned"] for k in res},
         "fp32_long_march_note": "over the 1.06M-step 256x1024 march to 2 cm the FP32 policy (round-to-nearest "
                                 "state) ends 3.9e-2 from FP64, mixed 4.0e-5 (profiles/accuracy_256x1024_full_march.md)",
         "field_rel_l2_vs_fp64": err or None,
@@ -689,12 +703,12 @@ def main():
         "phase_gradient": grad,
         "kernel_ms_per_step": {p: res[p]["kernel_ms"] for p in res},
         "roofline": roof,
-        "roofline_qdht": roofline("qdht", km["qdht_back"], R, T, "mixed", "k_qdht_bf16x6_t32<false>", cols),
-        "roofline_qdht_lstep": roofline("qdht", km["qdht_lstep"], R, T, "mixed", "k_qdht_bf16x3_t32<true>", cols),
+        "roofline_qdht": roofline("qdht", km["qdht_back"], R, T, "mixed", t32_kernel(6, False), cols),
+        "roofline_qdht_lstep": roofline("qdht", km["qdht_lstep"], R, T, "mixed", t32_kernel(3, True), cols),
         "roofline_qdht_fp32": roofline("qdht", res["fp32"]["kernel_ms"]["qdht_back"], R, T, "fp32",
                                        "k_qdht_f32<false, 32>", res["fp32"]["qdht_columns"]),
-        "roofline_qdht_all_columns": roofline("qdht", full["mixed"]["kernel_ms"]["qdht_lstep"], R, T, "mixed",
-                                              "k_qdht_bf16x3_t128<true>"),
+        "roofline_qdht_all_columns": roofline("qdht", full["mixed"]["kernel_ms"]["qdht_back"], R, T, "mixed",
+                                              t32_kernel(6, False)),
         "traffic_profile_current": traffic_profile_current(),
         "roofline_nl": roofline("nl", km["nl_rk4"], R, T, "mixed", "k_nl_rk4<PolMix>"),
         "roofline_nl_valu": valu_roofline("k_nl_rk4<PolMix>", km["nl_rk4"]),

@@ -59,6 +59,7 @@ struct QCols {
     int nzt;
     int zero_dead;      // host-driven launches: clear the skipped blocks of the output
     int split3;         // mixed: operands in three bf16 parts, six products (FP32-level accuracy)
+    int ks;             // mixed: K groups per narrow-tile workgroup, 1 or 2 (same bits; 0: 2)
 };
 
 // Scalars and table pointers the NL kernels read.  Table pointers refer to
@@ -181,7 +182,7 @@ int launch_b2_split(Prec p, const void* b2, void* fk, const void* gfac, size_t n
 // the bookkeeping after the linear half of n runs
 int launch_nl_rk4_batch(Prec p, int nt, int nl_e, bool mfma_fft, const NLRun* runs, const int2* items, int nitems,
                         hipStream_t st);
-int launch_qdht_batch(Prec p, bool lstep, const GemmRun* runs, const int* brun, int nblocks, hipStream_t st);
+int launch_qdht_batch(Prec p, bool lstep, const GemmRun* runs, const int* brun, int nblocks, hipStream_t st, int ks = 0);
 int launch_controller_batch(const CtlRun* runs, int n, hipStream_t st);
 int launch_finish_batch(DevCtl* ctls, int n, hipStream_t st);
 // blocks of one run's QDHT in a batched launch (the single-run grid of launch_qdht)

@@ -125,10 +125,18 @@ struct gs_plan {
     gs::QCols qcols(bool zero_dead, bool back) const
     {
         const int s3 = (back ? split3_back : split3_fwd) ? 1 : 0;
-        if (!qskip) return gs::QCols{nullptr, 0, nullptr, 0, 0, s3};
-        return gs::QCols{ctile.as<int>(), nct, ztile.as<int>(), nzt, zero_dead ? 1 : 0, s3};
+        if (!qskip) return gs::QCols{nullptr, 0, nullptr, 0, 0, s3, qks[1]};
+        return gs::QCols{ctile.as<int>(), nct, ztile.as<int>(), nzt, zero_dead ? 1 : 0, s3, qks[0]};
     }
-    bool march_cols(bool back) const { return qskip || (back ? split3_back : split3_fwd); }
+    // GEMM tile shapes (autotune_qdht; every choice gives the same bits): K groups
+    // of the mixed narrow tiles for the column-skipping [0] and all-column [1]
+    // march, and whether FP32 all-column transforms use the narrow 32-column tiles
+    int qks[2] = {2, 2};
+    bool f32_narrow_full = false;
+    // the march's transforms go through the column-list kernels when columns are
+    // skipped, and always in mixed precision (its narrow tiles sum in their own
+    // order, so the all-column march matches the skipping one bit for bit)
+    bool march_cols(bool) const { return qskip || prec == gs::kMIXED || (prec == gs::kFP32 && f32_narrow_full); }
 
     double ZSTEP = 0, CURRENT_Z = 0;
     long STEP_N = 0, n_reject = 0;
@@ -579,6 +587,75 @@ void autotune_nl(gs_plan* p)
     cache[key] = best;
 }
 
+// ---- autotuner of the QDHT tile shapes
+// Tile shapes that give the same bits, timed on the plan's own operator and
+// initial field (FIELD -> BIGBUFFER1, cleared afterwards) by the first plan of
+// a (precision, N_R, N_T) in the process: mixed, the narrow split-BF16 tiles
+// with one or two K groups per workgroup (256 or 512 threads), separately for
+// the column-skipping and the all-column transform back; FP32, 64- or
+// 32-column tiles for all-column transforms.  The default (two groups; 64
+// columns) is replaced only by a clear (3 %) win.  GS_QDHT_KSPLIT fixes the
+// mixed choice for the whole process.
+void autotune_qdht(gs_plan* p)
+{
+    static std::mutex mu;
+    static std::map<std::tuple<int, int, int>, std::tuple<int, int, bool>> cache;
+    if (p->prec == gs::kFP64) return;
+    const auto key = std::make_tuple((int)p->prec, p->s.NR, p->s.NT);
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            std::tie(p->qks[0], p->qks[1], p->f32_narrow_full) = it->second;
+            return;
+        }
+    }
+    hipEvent_t e0, e1;
+    HCK(hipEventCreate(&e0));
+    HCK(hipEventCreate(&e1));
+    // candidates alternate (warm-up round first); each keeps its best time
+    auto pick = [&](auto&& launch) {
+        float tmin[2] = {1e30f, 1e30f};
+        for (int rep = 0; rep < 5; ++rep)
+            for (int c = 0; c < 2; ++c) {
+                HCK(hipEventRecord(e0, p->st));
+                launch(c);
+                HCK(hipEventRecord(e1, p->st));
+                sync(p);
+                float t = 0;
+                HCK(hipEventElapsedTime(&t, e0, e1));
+                if (rep > 0) tmin[c] = std::min(tmin[c], t);
+            }
+        return tmin[1] < 0.97f * tmin[0] ? 1 : 0;
+    };
+    const int rp = p->rp, nt = p->s.NT;
+    if (p->prec == gs::kMIXED) {
+        const int s3 = p->split3_back ? 1 : 0;
+        for (int mode = 0; mode < 2; ++mode) {     // 0: the live column blocks, 1: every column
+            if (mode == 0 && p->nct == 0) continue;
+            const int cand[2] = {2, 1};
+            const int c = pick([&](int c) {
+                const gs::QCols qc = mode == 0 ? gs::QCols{p->ctile.as<int>(), p->nct, p->ztile.as<int>(), p->nzt, 0, s3, cand[c]}
+                                               : gs::QCols{nullptr, 0, nullptr, 0, 0, s3, cand[c]};
+                LCK(gs::launch_qdht(p->prec, rp, nt, p->amat.p, p->field.p, p->b1.p, nullptr, p->st, nullptr, &qc));
+            });
+            p->qks[mode] = cand[c];
+        }
+    } else {
+        const gs::QCols narrow{nullptr, 0, nullptr, 0, 0, 0, 0};
+        p->f32_narrow_full = pick([&](int c) {
+            LCK(gs::launch_qdht(p->prec, rp, nt, p->amat.p, p->field.p, p->b1.p, nullptr, p->st, nullptr,
+                                c ? &narrow : nullptr));
+        }) == 1;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HCK(hipMemsetAsync(p->b1.p, 0, p->field_bytes(), p->st));
+    sync(p);
+    std::lock_guard<std::mutex> g(mu);
+    cache[key] = std::make_tuple(p->qks[0], p->qks[1], p->f32_narrow_full);
+}
+
 // ---- one NL attempt + controller pieces
 void nl_step(gs_plan* p, double dz, double* maxA2, double* maxNL2)
 {
@@ -917,6 +994,7 @@ struct BatchMarch {
         bool lstep;
         DevBuf runs, brun;
         int nblocks;
+        int ks;
     };
     std::vector<NLGroup> nlg;
     std::vector<GemmLaunch> gl;
@@ -983,7 +1061,7 @@ struct BatchMarch {
                     runs.push_back(g);
                 }
                 if (runs.empty()) continue;
-                gl.push_back(GemmLaunch{pr, lstep, DevBuf(), DevBuf(), (int)brun.size()});
+                gl.push_back(GemmLaunch{pr, lstep, DevBuf(), DevBuf(), (int)brun.size(), runs[0].qc.ks});
                 upload(gl.back().runs, runs.data(), sizeof(gs::GemmRun) * runs.size());
                 upload(gl.back().brun, brun.data(), sizeof(int) * brun.size());
             }
@@ -1020,7 +1098,7 @@ struct BatchMarch {
         for (bool lstep : {true, false})
             for (auto& g : gl)
                 if (g.lstep == lstep)
-                    LCK(gs::launch_qdht_batch(g.prec, lstep, g.runs.as<gs::GemmRun>(), g.brun.as<int>(), g.nblocks, st));
+                    LCK(gs::launch_qdht_batch(g.prec, lstep, g.runs.as<gs::GemmRun>(), g.brun.as<int>(), g.nblocks, st, g.ks));
         LCK(gs::launch_finish_batch(ctls.as<gs::DevCtl>(), n, st));
     }
     void launch() { HCK(hipGraphLaunch(exec, st)); }
@@ -1050,6 +1128,7 @@ int gs_plan_create(const char* cfg_path, int nover, const char* const* over, int
         p->s = gs::build_setup(cfg, workdir ? workdir : "");
         build_plan(p.get());
         autotune_nl(p.get());
+        autotune_qdht(p.get());
         *out = p.release();
         return 0;
     });
@@ -1117,6 +1196,8 @@ static int get_scalar(gs_plan* p, const char* name, double* v)
     else if (n == "qS") *v = s.qS;
     else if (n == "precision") *v = p->prec;
     else if (n == "QDHT_SKIP_COLUMNS") *v = p->qskip ? 1 : 0;
+    else if (n == "QDHT_K_GROUPS") *v = p->qks[p->qskip ? 0 : 1];
+    else if (n == "QDHT_F32_NARROW_FULL") *v = p->f32_narrow_full ? 1 : 0;
     else if (n == "SCAN_FALLBACKS") {
         // line launches whose plasma scan needed the chunk-sequential fallback, since plan creation
         gs::DevStatus h;
@@ -1152,6 +1233,20 @@ int gs_set_scalar(gs_plan* p, const char* name, double v)
             p->slot_exec = nullptr;
         }
         p->qskip = on;
+    }
+    else if (n == "QDHT_K_GROUPS" || n == "QDHT_F32_NARROW_FULL") {
+        // the march's GEMM tile shape (normally autotuned; the same bits either way):
+        // K groups of the mixed narrow tiles in the current column mode (1 or 2),
+        // or FP32 all-column transforms on the narrow tiles (0 or 1)
+        const bool groups = n == "QDHT_K_GROUPS";
+        if (groups ? !(v == 1 || v == 2) : !(v == 0 || v == 1))
+            return fail(GS_E_ARG, groups ? "QDHT_K_GROUPS must be 1 or 2" : "QDHT_F32_NARROW_FULL must be 0 or 1");
+        if (p->slot_exec) {
+            (void)hipGraphExecDestroy(p->slot_exec);
+            p->slot_exec = nullptr;
+        }
+        if (groups) p->qks[p->qskip ? 0 : 1] = (int)v;
+        else p->f32_narrow_full = v != 0;
     }
     else if (n == "ADJOINT_CHECKPOINT_BYTES") {
         if (v != v) return fail(GS_E_ARG, "ADJOINT_CHECKPOINT_BYTES must be a number");

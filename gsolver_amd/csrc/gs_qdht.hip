@@ -442,14 +442,30 @@ __device__ __forceinline__ void sstore_b(uint16_t* b, int oB, const float (&xr)[
 
 // NS = 2: split-BF16 3-pass (lo*hi, hi*lo, hi*hi); NS = 3: both operands in three
 // bf16 parts (to ~2^-24, the FP32 accuracy of the field), six products
-// (mid*mid, hi*rest, rest*hi, mid*hi, hi*mid, hi*hi, smallest first)
-template <bool LSTEP, int NS = 2>
+// (mid*mid, hi*rest, rest*hi, mid*hi, hi*mid, hi*hi, smallest first).
+// 128 rows x 32 complex columns per workgroup of 256 x KS threads.  Every
+// output is (sum over the first K/2 rows) + (sum over the last K/2 rows), each
+// half summed in k order: with KS = 1 one group of 4 waves sums both halves
+// in turn (the first kept aside), with KS = 2 two groups sum one half each and
+// add them through LDS, each group writing half of the rows.  Both give the
+// same bits, so KS is a free choice of tile shape (two waves per SIMD on the
+// few, ~1 per CU, tiles of a column-skipping transform).  sm holds
+// smem_t32<NS, KS>() elements.
+template <int NS, int KS>
+constexpr int smem_t32()
+{
+    // double-buffered B per group; the exchange of 2 x 16 x 256 floats reuses it
+    return 2 * KS * t32::buf_elems<NS>() > (KS - 1) * 2 * 16 * 256 * 2 ? 2 * KS * t32::buf_elems<NS>()
+                                                                       : (KS - 1) * 2 * 16 * 256 * 2;
+}
+template <bool LSTEP, int NS = 2, int KS = 1>
 __device__ __forceinline__ void qdht_t32_body(int bid, const uint16_t* __restrict__ Afr, const C2<float>* __restrict__ X,
-                                              C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, const QCols& qc)
+                                              C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, const QCols& qc,
+                                              uint16_t* __restrict__ sm)
 {
     using namespace t32;
+    static_assert(KS == 1 || KS == 2, "one or two K ranges");
     constexpr int BUF = buf_elems<NS>();
-    __shared__ __align__(16) uint16_t sm[2 * BUF];
     const DevCtl* ctl = la.ctl;
     if (ctl) {
         if (ctl->state != 0 || !ctl->accept) return;
@@ -465,12 +481,13 @@ __device__ __forceinline__ void qdht_t32_body(int bid, const uint16_t* __restric
     int mi, ni;
     tile_of(bid, mt, nct, &mi, &ni);
     const int m0 = mi * BM, n0 = col0_of(qc, ni, BNC);
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x & 255, lane = tid & 63, w = tid >> 6;   // within the K group
+    const int ks = KS == 1 ? 0 : (int)(threadIdx.x >> 8);                // the K group
     const int bcol = tid & 31, bq = tid >> 5;               // B loader: column, k quarter (4 rows)
     const int oB = bcol * LD + 4 * bq;                       // + plane * PLANE
     const int rB = (lane & 31) * LD + 8 * (lane >> 5);      // + plane * PLANE + 16 kk
-    const C2<float>* gX = X + (size_t)(4 * bq) * NC + n0 + bcol;
-    const int nk = K / BK;
+    const int nka = K / BK, nk = nka / KS, kb = ks * nk;      // K steps: all, per group, group's first
+    const C2<float>* gX = X + (size_t)(4 * bq + kb * BK) * NC + n0 + bcol;
 
     v16f acc[2];    // [re, im]
 #pragma unroll
@@ -520,21 +537,32 @@ __device__ __forceinline__ void qdht_t32_body(int bid, const uint16_t* __restric
         }
     };
 
-    uint16_t* const buf0 = sm;
-    uint16_t* const buf1 = sm + BUF;
+    uint16_t* const buf0 = sm + ks * 2 * BUF;
+    uint16_t* const buf1 = buf0 + BUF;
     float xr[4], xi[4], nxr[4], nxi[4];
     v8bf a0[2][NS], a1[2][NS];
+    v16f first[2];     // KS = 1: the first half's sums
     gload_b(gX, NC, 0, nk, xr, xi);
     sstore_b<NS>(buf0, oB, xr, xi);
     gload_b(gX, NC, 1, nk, xr, xi);
-    gload_a<NS>(Afr, K, mi, nk, 0, w, lane, a0);
+    gload_a<NS>(Afr, K, mi, nka, kb, w, lane, a0);
     __syncthreads();
     OpsB ops;
     for (int k = 0; k < nk; k += 2) {
+        if constexpr (KS == 1) {
+            if (k == nk / 2) {
+#pragma unroll
+                for (int pt = 0; pt < 2; ++pt) {
+                    first[pt] = acc[pt];
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) acc[pt][r] = 0.f;
+                }
+            }
+        }
         read_b(buf0, ops);
         __builtin_amdgcn_sched_barrier(0);
         gload_b(gX, NC, k + 2, nk, nxr, nxi);
-        gload_a<NS>(Afr, K, mi, nk, k + 1, w, lane, a1);
+        gload_a<NS>(Afr, K, mi, nka, kb + k + 1, w, lane, a1);
         mfma_ops(a0, ops);
         sstore_b<NS>(buf1, oB, xr, xi);
         schedule();
@@ -542,16 +570,53 @@ __device__ __forceinline__ void qdht_t32_body(int bid, const uint16_t* __restric
         read_b(buf1, ops);
         __builtin_amdgcn_sched_barrier(0);
         gload_b(gX, NC, k + 3, nk, xr, xi);
-        gload_a<NS>(Afr, K, mi, nk, k + 2, w, lane, a0);
+        gload_a<NS>(Afr, K, mi, nka, kb + k + 2, w, lane, a0);
         mfma_ops(a1, ops);
         sstore_b<NS>(buf0, oB, nxr, nxi);
         schedule();
         __syncthreads();
     }
+    if constexpr (KS == 1) {
+#pragma unroll
+        for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[pt][r] = first[pt][r] + acc[pt][r];
+    }
+    if constexpr (KS == 2) {
+        // group g keeps rows h = g of the lane's tile; it hands the other half of its
+        // partial sums to the partner group (lane-contiguous, conflict-free) and adds
+        // the partner's: y = range 0 + range 1 in both groups
+        float* xch = reinterpret_cast<float*>(sm);
+        // (fixed register indices in both branches: no dynamic indexing of acc)
+        if (ks == 0) {
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) xch[((16 + pt * 8 + j) * 256) + tid] = acc[pt][8 + j];
+        } else {
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) xch[((pt * 8 + j) * 256) + tid] = acc[pt][j];
+        }
+        __syncthreads();
+        if (ks == 0) {
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) acc[pt][j] += xch[((pt * 8 + j) * 256) + tid];
+        } else {
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) acc[pt][8 + j] += xch[((16 + pt * 8 + j) * 256) + tid];
+        }
+    }
     const int col = n0 + (lane & 31);
     const double amp = LSTEP ? lstep_amp<PolMix>(col, la) : 1.0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
+        if (KS == 2 && h != ks) continue;
         size_t o[8];
         float yr[8], yi[8];
 #pragma unroll
@@ -570,17 +635,19 @@ __device__ __forceinline__ void qdht_t32_body(int bid, const uint16_t* __restric
         }
     }
 }
-template <bool LSTEP>
-__global__ __launch_bounds__(256) void k_qdht_bf16x3_t32(const uint16_t* __restrict__ Afr, const C2<float>* __restrict__ X,
-                                                         C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, QCols qc)
+template <bool LSTEP, int KS>
+__global__ __launch_bounds__(256 * KS) void k_qdht_bf16x3_t32(const uint16_t* __restrict__ Afr, const C2<float>* __restrict__ X,
+                                                              C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, QCols qc)
 {
-    qdht_t32_body<LSTEP, 2>(blockIdx.x, Afr, X, Y, K, NC, la, qc);
+    __shared__ __align__(16) uint16_t sm[smem_t32<2, KS>()];
+    qdht_t32_body<LSTEP, 2, KS>(blockIdx.x, Afr, X, Y, K, NC, la, qc, sm);
 }
-template <bool LSTEP>
-__global__ __launch_bounds__(256) void k_qdht_bf16x6_t32(const uint16_t* __restrict__ Afr, const C2<float>* __restrict__ X,
-                                                         C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, QCols qc)
+template <bool LSTEP, int KS>
+__global__ __launch_bounds__(256 * KS) void k_qdht_bf16x6_t32(const uint16_t* __restrict__ Afr, const C2<float>* __restrict__ X,
+                                                              C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, QCols qc)
 {
-    qdht_t32_body<LSTEP, 3>(blockIdx.x, Afr, X, Y, K, NC, la, qc);
+    __shared__ __align__(16) uint16_t sm[smem_t32<3, KS>()];
+    qdht_t32_body<LSTEP, 3, KS>(blockIdx.x, Afr, X, Y, K, NC, la, qc, sm);
 }
 
 // FP64: tile 64 rows x 32 complex columns, K step 16, 4 waves (2x2)
@@ -754,14 +821,16 @@ __global__ void k_finish(DevCtl* ctl) { finish_body(ctl); }
 // run, and blocks past a run's count (padding) return at once.  The bodies
 // are those of the single-run kernels, so every run's results are bit for bit
 // those of its own march.
-template <bool LSTEP>
-__global__ __launch_bounds__(256) void k_qdht_t32_batch(const GemmRun* __restrict__ runs, const int* __restrict__ brun)
+template <bool LSTEP, int KS>
+__global__ __launch_bounds__(256 * KS) void k_qdht_t32_batch(const GemmRun* __restrict__ runs, const int* __restrict__ brun)
 {
+    constexpr int SM = smem_t32<3, KS>() > smem_t32<2, KS>() ? smem_t32<3, KS>() : smem_t32<2, KS>();
+    __shared__ __align__(16) uint16_t sm[SM];
     const GemmRun& g = runs[brun[blockIdx.x]];
     const int bid = (int)blockIdx.x - g.b0;
     if (bid >= g.nb) return;
-    if (g.qc.split3) qdht_t32_body<LSTEP, 3>(bid, (const uint16_t*)g.A, (const C2<float>*)g.X, (C2<float>*)g.Y, g.K, g.NC, g.la, g.qc);
-    else qdht_t32_body<LSTEP, 2>(bid, (const uint16_t*)g.A, (const C2<float>*)g.X, (C2<float>*)g.Y, g.K, g.NC, g.la, g.qc);
+    if (g.qc.split3) qdht_t32_body<LSTEP, 3, KS>(bid, (const uint16_t*)g.A, (const C2<float>*)g.X, (C2<float>*)g.Y, g.K, g.NC, g.la, g.qc, sm);
+    else qdht_t32_body<LSTEP, 2, KS>(bid, (const uint16_t*)g.A, (const C2<float>*)g.X, (C2<float>*)g.Y, g.K, g.NC, g.la, g.qc, sm);
 }
 template <bool LSTEP>
 __global__ __launch_bounds__(256) void k_qdht_f32_batch(const GemmRun* __restrict__ runs, const int* __restrict__ brun)
@@ -797,7 +866,18 @@ int qdht_blocks(Prec p, int rp, int nt, const QCols* cols, bool back, bool gated
     return mt * cols->nct + ((back && (cols->zero_dead || gated)) ? mt * ((cols->nzt + 7) / 8) : 0);
 }
 
-int launch_qdht_batch(Prec p, bool lstep, const GemmRun* runs, const int* brun, int nblocks, hipStream_t st)
+// K groups of the narrow split-BF16 tiles: the caller's choice (the plan's
+// autotuned one), unless GS_QDHT_KSPLIT (1 or 2) fixes it; both give the same bits
+static int t32_groups(int want)
+{
+    static const int env = [] {
+        const char* e = getenv("GS_QDHT_KSPLIT");
+        return e ? (atoi(e) == 1 ? 1 : 2) : 0;
+    }();
+    return env ? env : (want == 1 ? 1 : 2);
+}
+
+int launch_qdht_batch(Prec p, bool lstep, const GemmRun* runs, const int* brun, int nblocks, hipStream_t st, int ks)
 {
     const dim3 grid(nblocks), blk(256);
     if (p == kFP64) {
@@ -806,9 +886,13 @@ int launch_qdht_batch(Prec p, bool lstep, const GemmRun* runs, const int* brun, 
     } else if (p == kFP32) {
         if (lstep) hipLaunchKernelGGL(k_qdht_f32_batch<true>, grid, blk, 0, st, runs, brun);
         else hipLaunchKernelGGL(k_qdht_f32_batch<false>, grid, blk, 0, st, runs, brun);
+    } else if (t32_groups(ks) == 2) {
+        const dim3 b2(512);
+        if (lstep) hipLaunchKernelGGL((k_qdht_t32_batch<true, 2>), grid, b2, 0, st, runs, brun);
+        else hipLaunchKernelGGL((k_qdht_t32_batch<false, 2>), grid, b2, 0, st, runs, brun);
     } else {
-        if (lstep) hipLaunchKernelGGL(k_qdht_t32_batch<true>, grid, blk, 0, st, runs, brun);
-        else hipLaunchKernelGGL(k_qdht_t32_batch<false>, grid, blk, 0, st, runs, brun);
+        if (lstep) hipLaunchKernelGGL((k_qdht_t32_batch<true, 1>), grid, blk, 0, st, runs, brun);
+        else hipLaunchKernelGGL((k_qdht_t32_batch<false, 1>), grid, blk, 0, st, runs, brun);
     }
     HIPCHK(hipGetLastError());
     return 0;
@@ -911,10 +995,13 @@ int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* in, void* 
         const int mt = rp / 128;
         if (cols) {
             const dim3 grid(mt * ncb + zblocks(mt));
-            auto k3t = k_qdht_bf16x3_t32<true>, k3f = k_qdht_bf16x3_t32<false>;
-            auto k6t = k_qdht_bf16x6_t32<true>, k6f = k_qdht_bf16x6_t32<false>;
+            const int ks = t32_groups(qc.ks);
+            auto k3t = ks == 2 ? k_qdht_bf16x3_t32<true, 2> : k_qdht_bf16x3_t32<true, 1>;
+            auto k3f = ks == 2 ? k_qdht_bf16x3_t32<false, 2> : k_qdht_bf16x3_t32<false, 1>;
+            auto k6t = ks == 2 ? k_qdht_bf16x6_t32<true, 2> : k_qdht_bf16x6_t32<true, 1>;
+            auto k6f = ks == 2 ? k_qdht_bf16x6_t32<false, 2> : k_qdht_bf16x6_t32<false, 1>;
             auto kf = qc.split3 ? (ls ? k6t : k6f) : (ls ? k3t : k3f);
-            hipLaunchKernelGGL(kf, grid, dim3(256), 0, st, (const uint16_t*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
+            hipLaunchKernelGGL(kf, grid, dim3(256 * ks), 0, st, (const uint16_t*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
         } else {
             const dim3 grid(mt * (nt / 128));
             auto kt = k_qdht_bf16x3_t128<true>;

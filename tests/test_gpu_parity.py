@@ -485,16 +485,19 @@ def test_mixed_linear_phase_drift_without_nonlinearity(tmp_path):
     assert e_32 > 0.5 * drift
 
 
+@pytest.mark.parametrize("nr", [64, 256])
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
-def test_column_skipping_equals_full_transforms(tmp_path, prec):
+def test_column_skipping_equals_full_transforms(tmp_path, prec, nr):
     """The march's two QDHTs compute only the 32-column blocks that hold a point
     the linear step keeps (propagator.cu:601-603 zeroes the others): 296 of the
     512 columns here.  Against the same march with every column transformed
     (QDHT_SKIP_COLUMNS 0), three fixed steps and then an adaptive march leave
     FIELD and BIGBUFFER2 identical bit for bit, the skipped columns of FIELD
     cleared by the first QDHT back in both the host-driven and the
-    device-driven step."""
-    cfg = write_cfg(tmp_path / "c.txt", **HOT)
+    device-driven step.  At N_R = 256 both halves of the K range of the
+    split-BF16 tiles hold non-zero operator rows (at 64 the padding rows are
+    zero)."""
+    cfg = write_cfg(tmp_path / "c.txt", **dict(HOT, N_R=str(nr)))
     out = []
     for skip in (0, 1):
         p = gsa.Plan(cfg, precision=prec)
@@ -519,6 +522,44 @@ def test_column_skipping_equals_full_transforms(tmp_path, prec):
     for a, b in zip(full, skip):
         assert np.array_equal(a, b)
     assert np.all(full[0][:, dead] == 0) and np.all(full[2][:, dead] == 0)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+def test_qdht_tile_shapes_give_identical_bits(tmp_path, prec):
+    """The GEMM tile autotuner chooses among tile shapes that give the same
+    bits: mixed, the narrow split-BF16 tiles with one or two K groups per
+    workgroup (each output = first-half sum + second-half sum either way), in
+    the column-skipping and the all-column march; FP32, 64- or 32-column tiles
+    for all-column transforms.  Three fixed steps and a short adaptive march
+    leave FIELD and BIGBUFFER2 identical for every choice (N_R = 256: both K
+    halves hold operator rows)."""
+    cfg = write_cfg(tmp_path / "c.txt", **dict(HOT, N_R="256"))
+    if prec == "mixed":
+        cases = [(skip, "QDHT_K_GROUPS", v) for skip in (1, 0) for v in (1, 2)]
+    else:
+        cases = [(0, "QDHT_F32_NARROW_FULL", v) for v in (0, 1)]
+    got = {}
+    for skip, name, v in cases:
+        p = gsa.Plan(cfg, precision=prec)
+        assert p.scalar(name) in ((1, 2) if name == "QDHT_K_GROUPS" else (0, 1))
+        p.set_scalar("QDHT_SKIP_COLUMNS", skip)
+        p.set_scalar(name, v)
+        assert p.scalar(name) == v
+        p.fixed_steps(1e-6, 3, timings=False)
+        f3 = p.get()
+        p.march(str(tmp_path / ("m%d%d" % (skip, v))))
+        got[(skip, v)] = (f3, p.get(), p.get(gsa.B2), p.scalar("STEP_N"))
+        p.close()
+    ref = got[cases[0][0], cases[0][2]]
+    for k, g in got.items():
+        for a, b in zip(ref, g):
+            assert np.array_equal(a, b), k
+    with pytest.raises(RuntimeError):
+        p = gsa.Plan(cfg, precision=prec)
+        try:
+            p.set_scalar("QDHT_K_GROUPS" if prec == "mixed" else "QDHT_F32_NARROW_FULL", 3)
+        finally:
+            p.close()
 
 
 def test_mixed_three_part_split_qdht(tmp_path, monkeypatch):
