@@ -41,7 +41,8 @@ def run_precision(gsa, cfg, prec, steps, warmup, dz, skip=True):
     p = gsa.Plan(cfg, precision=prec)
     p.set_scalar("QDHT_SKIP_COLUMNS", 1 if skip else 0)
     live_cols = 32 * p.scalar("QDHT_LIVE_BLOCKS") if skip else p.T
-    tiles = {"nl_samples_per_thread": int(p.scalar("NL_SAMPLES_PER_THREAD"))}
+    tiles = {"nl_samples_per_thread": int(p.scalar("NL_SAMPLES_PER_THREAD")),
+             "qdht_xcd_grouping": "by row tile" if p.scalar("QDHT_TILE_ORDER") else "by column tile"}
     if prec == "mixed":
         tiles["qdht_k_groups"] = int(p.scalar("QDHT_K_GROUPS"))
     elif prec == "fp32" and not skip:
@@ -301,10 +302,12 @@ def plane_march(gsa, workloads, work, prec, z_plane, state_dir, resume, time_lim
     march longer than one session runs in pieces and ends where an
     uninterrupted one would (FP64 exactly).  At the plane, the per-line
     fluence TSTEP sum_t |E|^2, peak intensity and peak plasma density
-    (gs_line_diagnostics) are saved with the field."""
+    (gs_line_diagnostics) are saved with the field.  prec "fp64_perturbed" is
+    the FP64 march from FIELD and BIGBUFFER2 with seeded relative noise 1e-12
+    (how far a rounding-level perturbation grows over the same march)."""
     import numpy as np
     cfg = workloads.write_config(os.path.join(work, "plane"), "c1", N_Z=2, Z_MAX=z_plane)
-    p = gsa.Plan(cfg, precision=prec)
+    p = gsa.Plan(cfg, precision="fp64" if prec == "fp64_perturbed" else prec)
     om = p.table("omega")
     live = (om >= p.scalar("OMEGA_MIN")) & (om <= p.scalar("OMEGA_MAX"))
     cols = np.nonzero(np.repeat(np.add.reduceat(live, np.arange(0, p.T, 32)) > 0, 32))[0]
@@ -320,6 +323,11 @@ def plane_march(gsa, workloads, work, prec, z_plane, state_dir, resume, time_lim
         p.set_scalar("ZSTEP", float(st["dz"]))
         p.set_scalar("n_Z", 1)
         steps0, rej0 = int(st["steps"]), int(st["rejected"])
+    elif prec == "fp64_perturbed":
+        rng = np.random.default_rng(4242)
+        for buf in (gsa.FIELD, gsa.B2):
+            f = p.get(buf)
+            p.put(f * (1 + 1e-12 * rng.standard_normal(f.shape)), buf)
     t0 = time.perf_counter()
     done = False
     while not done and time.perf_counter() - t0 < time_limit:

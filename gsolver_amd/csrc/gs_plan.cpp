@@ -125,18 +125,22 @@ struct gs_plan {
     gs::QCols qcols(bool zero_dead, bool back) const
     {
         const int s3 = (back ? split3_back : split3_fwd) ? 1 : 0;
-        if (!qskip) return gs::QCols{nullptr, 0, nullptr, 0, 0, s3, qks[1]};
-        return gs::QCols{ctile.as<int>(), nct, ztile.as<int>(), nzt, zero_dead ? 1 : 0, s3, qks[0]};
+        if (!qskip)
+            return gs::QCols{nullptr, 0, nullptr, 0, 0, s3, qks[1], qorder[1], prec == gs::kFP32 && !f32_narrow_full ? 1 : 0};
+        return gs::QCols{ctile.as<int>(), nct, ztile.as<int>(), nzt, zero_dead ? 1 : 0, s3, qks[0], qorder[0], 0};
     }
-    // GEMM tile shapes (autotune_qdht; every choice gives the same bits): K groups
-    // of the mixed narrow tiles for the column-skipping [0] and all-column [1]
-    // march, and whether FP32 all-column transforms use the narrow 32-column tiles
+    // GEMM tile shapes (autotune_qdht; every choice gives the same bits), for the
+    // column-skipping [0] and the all-column [1] march: K groups of the mixed
+    // narrow tiles, the tile order over the XCDs, and whether FP32 all-column
+    // transforms use the narrow 32-column tiles
     int qks[2] = {2, 2};
+    int qorder[2] = {0, 0};
     bool f32_narrow_full = false;
     // the march's transforms go through the column-list kernels when columns are
     // skipped, and always in mixed precision (its narrow tiles sum in their own
     // order, so the all-column march matches the skipping one bit for bit)
-    bool march_cols(bool) const { return qskip || prec == gs::kMIXED || (prec == gs::kFP32 && f32_narrow_full); }
+    // the march's transforms always take the plan's column list and tile shape
+    bool march_cols(bool) const { return true; }
 
     double ZSTEP = 0, CURRENT_Z = 0;
     long STEP_N = 0, n_reject = 0;
@@ -590,70 +594,88 @@ void autotune_nl(gs_plan* p)
 // ---- autotuner of the QDHT tile shapes
 // Tile shapes that give the same bits, timed on the plan's own operator and
 // initial field (FIELD -> BIGBUFFER1, cleared afterwards) by the first plan of
-// a (precision, N_R, N_T) in the process: mixed, the narrow split-BF16 tiles
-// with one or two K groups per workgroup (256 or 512 threads), separately for
-// the column-skipping and the all-column transform back; FP32, 64- or
-// 32-column tiles for all-column transforms.  The default (two groups; 64
-// columns) is replaced only by a clear (3 %) win.  GS_QDHT_KSPLIT fixes the
-// mixed choice for the whole process.
+// a (precision, N_R, N_T) in the process, separately for the column-skipping
+// and the all-column transform (back, the one without the linear step):
+// the tile order over the XCDs (all row tiles of a column tile per XCD, or all
+// column tiles of a row tile) in every precision; mixed, one or two K groups
+// per narrow-tile workgroup (256 or 512 threads); FP32 all-column, 64- or
+// 32-column tiles.  The default (the first candidate) is replaced only by a
+// clear (3 %) win.  GS_QDHT_KSPLIT fixes the mixed K groups for the process.
+struct QShape {
+    int ks = 2, order = 0;
+    bool narrow = false;
+};
 void autotune_qdht(gs_plan* p)
 {
     static std::mutex mu;
-    static std::map<std::tuple<int, int, int>, std::tuple<int, int, bool>> cache;
-    if (p->prec == gs::kFP64) return;
+    static std::map<std::tuple<int, int, int>, std::pair<QShape, QShape>> cache;
     const auto key = std::make_tuple((int)p->prec, p->s.NR, p->s.NT);
+    auto apply = [&](const QShape& a, const QShape& b) {
+        p->qks[0] = a.ks;
+        p->qorder[0] = a.order;
+        p->qks[1] = b.ks;
+        p->qorder[1] = b.order;
+        p->f32_narrow_full = b.narrow;
+    };
     {
         std::lock_guard<std::mutex> g(mu);
         auto it = cache.find(key);
         if (it != cache.end()) {
-            std::tie(p->qks[0], p->qks[1], p->f32_narrow_full) = it->second;
+            apply(it->second.first, it->second.second);
             return;
         }
     }
     hipEvent_t e0, e1;
     HCK(hipEventCreate(&e0));
     HCK(hipEventCreate(&e1));
-    // candidates alternate (warm-up round first); each keeps its best time
-    auto pick = [&](auto&& launch) {
-        float tmin[2] = {1e30f, 1e30f};
+    const int rp = p->rp, nt = p->s.NT;
+    const void* A = p->prec == gs::kFP64 ? p->at64.p : p->amat.p;
+    const int s3 = p->prec == gs::kMIXED && p->split3_back ? 1 : 0;
+    QShape best[2];
+    for (int mode = 0; mode < 2; ++mode) {     // 0: the live column blocks, 1: every column
+        if (mode == 0 && p->nct == 0) continue;
+        std::vector<QShape> cand;
+        for (int order : {0, 1}) {
+            if (p->prec == gs::kMIXED) {
+                for (int ks : {2, 1}) cand.push_back(QShape{ks, order, false});
+            } else if (p->prec == gs::kFP32 && mode == 1) {
+                for (bool nar : {false, true}) cand.push_back(QShape{2, order, nar});
+            } else {
+                cand.push_back(QShape{2, order, false});
+            }
+        }
+        // candidates alternate (warm-up round first); each keeps its best time
+        std::vector<float> tmin(cand.size(), 1e30f);
         for (int rep = 0; rep < 5; ++rep)
-            for (int c = 0; c < 2; ++c) {
+            for (size_t c = 0; c < cand.size(); ++c) {
+                const QShape& q = cand[c];
+                const gs::QCols qc = mode == 0
+                    ? gs::QCols{p->ctile.as<int>(), p->nct, p->ztile.as<int>(), p->nzt, 0, s3, q.ks, q.order, 0}
+                    : gs::QCols{nullptr, 0, nullptr, 0, 0, s3, q.ks, q.order, p->prec == gs::kFP32 && !q.narrow ? 1 : 0};
                 HCK(hipEventRecord(e0, p->st));
-                launch(c);
+                LCK(gs::launch_qdht(p->prec, rp, nt, A, p->field.p, p->b1.p, nullptr, p->st, nullptr, &qc));
                 HCK(hipEventRecord(e1, p->st));
                 sync(p);
                 float t = 0;
                 HCK(hipEventElapsedTime(&t, e0, e1));
                 if (rep > 0) tmin[c] = std::min(tmin[c], t);
             }
-        return tmin[1] < 0.97f * tmin[0] ? 1 : 0;
-    };
-    const int rp = p->rp, nt = p->s.NT;
-    if (p->prec == gs::kMIXED) {
-        const int s3 = p->split3_back ? 1 : 0;
-        for (int mode = 0; mode < 2; ++mode) {     // 0: the live column blocks, 1: every column
-            if (mode == 0 && p->nct == 0) continue;
-            const int cand[2] = {2, 1};
-            const int c = pick([&](int c) {
-                const gs::QCols qc = mode == 0 ? gs::QCols{p->ctile.as<int>(), p->nct, p->ztile.as<int>(), p->nzt, 0, s3, cand[c]}
-                                               : gs::QCols{nullptr, 0, nullptr, 0, 0, s3, cand[c]};
-                LCK(gs::launch_qdht(p->prec, rp, nt, p->amat.p, p->field.p, p->b1.p, nullptr, p->st, nullptr, &qc));
-            });
-            p->qks[mode] = cand[c];
-        }
-    } else {
-        const gs::QCols narrow{nullptr, 0, nullptr, 0, 0, 0, 0};
-        p->f32_narrow_full = pick([&](int c) {
-            LCK(gs::launch_qdht(p->prec, rp, nt, p->amat.p, p->field.p, p->b1.p, nullptr, p->st, nullptr,
-                                c ? &narrow : nullptr));
-        }) == 1;
+        size_t b = 0;
+        float tb = 0.97f * tmin[0];
+        for (size_t c = 1; c < cand.size(); ++c)
+            if (tmin[c] < tb) {
+                tb = tmin[c];
+                b = c;
+            }
+        best[mode] = cand[b];
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     HCK(hipMemsetAsync(p->b1.p, 0, p->field_bytes(), p->st));
     sync(p);
+    apply(best[0], best[1]);
     std::lock_guard<std::mutex> g(mu);
-    cache[key] = std::make_tuple(p->qks[0], p->qks[1], p->f32_narrow_full);
+    cache[key] = std::make_pair(best[0], best[1]);
 }
 
 // ---- one NL attempt + controller pieces
@@ -1197,6 +1219,7 @@ static int get_scalar(gs_plan* p, const char* name, double* v)
     else if (n == "precision") *v = p->prec;
     else if (n == "QDHT_SKIP_COLUMNS") *v = p->qskip ? 1 : 0;
     else if (n == "QDHT_K_GROUPS") *v = p->qks[p->qskip ? 0 : 1];
+    else if (n == "QDHT_TILE_ORDER") *v = p->qorder[p->qskip ? 0 : 1];
     else if (n == "QDHT_F32_NARROW_FULL") *v = p->f32_narrow_full ? 1 : 0;
     else if (n == "SCAN_FALLBACKS") {
         // line launches whose plasma scan needed the chunk-sequential fallback, since plan creation
@@ -1234,18 +1257,20 @@ int gs_set_scalar(gs_plan* p, const char* name, double v)
         }
         p->qskip = on;
     }
-    else if (n == "QDHT_K_GROUPS" || n == "QDHT_F32_NARROW_FULL") {
+    else if (n == "QDHT_K_GROUPS" || n == "QDHT_TILE_ORDER" || n == "QDHT_F32_NARROW_FULL") {
         // the march's GEMM tile shape (normally autotuned; the same bits either way):
         // K groups of the mixed narrow tiles in the current column mode (1 or 2),
-        // or FP32 all-column transforms on the narrow tiles (0 or 1)
+        // the tile order over the XCDs in the current column mode (0 or 1), or
+        // FP32 all-column transforms on the narrow tiles (0 or 1)
         const bool groups = n == "QDHT_K_GROUPS";
         if (groups ? !(v == 1 || v == 2) : !(v == 0 || v == 1))
-            return fail(GS_E_ARG, groups ? "QDHT_K_GROUPS must be 1 or 2" : "QDHT_F32_NARROW_FULL must be 0 or 1");
+            return fail(GS_E_ARG, n + (groups ? " must be 1 or 2" : " must be 0 or 1"));
         if (p->slot_exec) {
             (void)hipGraphExecDestroy(p->slot_exec);
             p->slot_exec = nullptr;
         }
         if (groups) p->qks[p->qskip ? 0 : 1] = (int)v;
+        else if (n == "QDHT_TILE_ORDER") p->qorder[p->qskip ? 0 : 1] = (int)v;
         else p->f32_narrow_full = v != 0;
     }
     else if (n == "ADJOINT_CHECKPOINT_BYTES") {

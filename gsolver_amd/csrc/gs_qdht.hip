@@ -22,15 +22,24 @@ namespace gs {
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// blockIdx -> tile, grouping all row tiles of a column tile on one XCD so the
-// B columns are reused from that XCD's L2 (dispatch is round-robin over 8 XCDs)
-__device__ __forceinline__ void tile_of(int b, int mt, int ntl, int* mi, int* ni)
+// blockIdx -> tile.  Dispatch is round-robin over the 8 XCDs, each with its own
+// L2: block b runs on XCD b & 7, which gets one contiguous run of tiles.
+// rows = false: all row tiles of a column tile on one XCD (its B columns are
+// read into that L2 once); rows = true: all column tiles of a row tile (its
+// rows of the operator A are), the better choice when A is larger than an L2
+// and the column tiles are few.  Same arithmetic either way.
+__device__ __forceinline__ void tile_of(int b, int mt, int ntl, int* mi, int* ni, bool rows = false)
 {
     const int tiles = mt * ntl;
     int t = b;
     if ((tiles & 7) == 0) t = (b & 7) * (tiles >> 3) + (b >> 3);
-    *mi = t % mt;
-    *ni = t / mt;
+    if (rows) {
+        *mi = t / ntl;
+        *ni = t % ntl;
+    } else {
+        *mi = t % mt;
+        *ni = t / mt;
+    }
 }
 
 // First column of column tile ni: the listed 32-column block of a
@@ -81,7 +90,7 @@ __device__ __forceinline__ void qdht_f32_body(int bid, const float* __restrict__
         return;
     }
     int mi, ni;
-    tile_of(bid, mt, nct, &mi, &ni);
+    tile_of(bid, mt, nct, &mi, &ni, qc.order != 0);
     const int m0 = mi * BM, n0 = col0_of(qc, ni, BNC);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = BNC == 64 ? (w >> 1) * 64 : w * 32, wn = BNC == 64 ? (w & 1) * 32 : 0;
@@ -479,7 +488,7 @@ __device__ __forceinline__ void qdht_t32_body(int bid, const uint16_t* __restric
         return;
     }
     int mi, ni;
-    tile_of(bid, mt, nct, &mi, &ni);
+    tile_of(bid, mt, nct, &mi, &ni, qc.order != 0);
     const int m0 = mi * BM, n0 = col0_of(qc, ni, BNC);
     const int tid = threadIdx.x & 255, lane = tid & 63, w = tid >> 6;   // within the K group
     const int ks = KS == 1 ? 0 : (int)(threadIdx.x >> 8);                // the K group
@@ -672,7 +681,7 @@ __device__ __forceinline__ void qdht_f64_body(int bid, const double* __restrict_
         return;
     }
     int mi, ni;
-    tile_of(bid, mt, nct, &mi, &ni);
+    tile_of(bid, mt, nct, &mi, &ni, qc.order != 0);
     const int m0 = mi * BM, n0 = col0_of(qc, ni, BNC);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = (w >> 1) * 32, wn = (w & 1) * 16;
@@ -981,7 +990,7 @@ int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* in, void* 
     } else if (p == kFP32) {
         if (rp % 128 || nt % 64) return -1001;
         const int mt = rp / 128;
-        if (cols) {
+        if (cols && (qc.ctile || !qc.wide)) {
             const dim3 grid(mt * ncb + zblocks(mt));
             if (ls) hipLaunchKernelGGL((k_qdht_f32<true, 32>), grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
             else hipLaunchKernelGGL((k_qdht_f32<false, 32>), grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);

@@ -524,42 +524,46 @@ def test_column_skipping_equals_full_transforms(tmp_path, prec, nr):
     assert np.all(full[0][:, dead] == 0) and np.all(full[2][:, dead] == 0)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
 def test_qdht_tile_shapes_give_identical_bits(tmp_path, prec):
     """The GEMM tile autotuner chooses among tile shapes that give the same
-    bits: mixed, the narrow split-BF16 tiles with one or two K groups per
-    workgroup (each output = first-half sum + second-half sum either way), in
-    the column-skipping and the all-column march; FP32, 64- or 32-column tiles
-    for all-column transforms.  Three fixed steps and a short adaptive march
-    leave FIELD and BIGBUFFER2 identical for every choice (N_R = 256: both K
-    halves hold operator rows)."""
+    bits: in every precision the tile order over the XCDs (QDHT_TILE_ORDER);
+    mixed, the narrow split-BF16 tiles with one or two K groups per workgroup
+    (each output = first-half sum + second-half sum either way); FP32, 64- or
+    32-column tiles for all-column transforms.  Three fixed steps and a short
+    adaptive march leave FIELD and BIGBUFFER2 identical for every choice, in
+    the column-skipping and the all-column march (N_R = 256: both K halves
+    hold operator rows)."""
     cfg = write_cfg(tmp_path / "c.txt", **dict(HOT, N_R="256"))
+    cases = [(skip, {"QDHT_TILE_ORDER": o}) for skip in (1, 0) for o in (0, 1)]
     if prec == "mixed":
-        cases = [(skip, "QDHT_K_GROUPS", v) for skip in (1, 0) for v in (1, 2)]
-    else:
-        cases = [(0, "QDHT_F32_NARROW_FULL", v) for v in (0, 1)]
-    got = {}
-    for skip, name, v in cases:
+        cases += [(skip, {"QDHT_K_GROUPS": g, "QDHT_TILE_ORDER": 1}) for skip in (1, 0) for g in (1, 2)]
+    elif prec == "fp32":
+        cases += [(0, {"QDHT_F32_NARROW_FULL": v, "QDHT_TILE_ORDER": 1}) for v in (0, 1)]
+    got = []
+    for skip, sets in cases:
         p = gsa.Plan(cfg, precision=prec)
-        assert p.scalar(name) in ((1, 2) if name == "QDHT_K_GROUPS" else (0, 1))
+        assert p.scalar("QDHT_TILE_ORDER") in (0, 1)
         p.set_scalar("QDHT_SKIP_COLUMNS", skip)
-        p.set_scalar(name, v)
-        assert p.scalar(name) == v
+        for name, v in sets.items():
+            p.set_scalar(name, v)
+            assert p.scalar(name) == v
         p.fixed_steps(1e-6, 3, timings=False)
         f3 = p.get()
-        p.march(str(tmp_path / ("m%d%d" % (skip, v))))
-        got[(skip, v)] = (f3, p.get(), p.get(gsa.B2), p.scalar("STEP_N"))
+        p.march(str(tmp_path / ("m%d" % len(got))))
+        got.append(((skip, sets), (f3, p.get(), p.get(gsa.B2), p.scalar("STEP_N"))))
         p.close()
-    ref = got[cases[0][0], cases[0][2]]
-    for k, g in got.items():
+    ref = got[0][1]
+    for k, g in got[1:]:
         for a, b in zip(ref, g):
             assert np.array_equal(a, b), k
-    with pytest.raises(RuntimeError):
-        p = gsa.Plan(cfg, precision=prec)
-        try:
-            p.set_scalar("QDHT_K_GROUPS" if prec == "mixed" else "QDHT_F32_NARROW_FULL", 3)
-        finally:
-            p.close()
+    p = gsa.Plan(cfg, precision=prec)
+    try:
+        for name, bad in (("QDHT_TILE_ORDER", 2), ("QDHT_K_GROUPS", 3), ("QDHT_F32_NARROW_FULL", 5)):
+            with pytest.raises(RuntimeError):
+                p.set_scalar(name, bad)
+    finally:
+        p.close()
 
 
 def test_mixed_three_part_split_qdht(tmp_path, monkeypatch):
