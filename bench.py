@@ -298,9 +298,9 @@ def plane_march(gsa, workloads, work, prec, z_plane, state_dir, resume, time_lim
     FIELD_k on the 32-column blocks the march transforms -- every other column
     of both is zero -- CURRENT_Z, ZSTEP and the step counts) is written to
     state_dir/<prec>_state.npz, and `resume` (None or the path of such a file)
-    continues from it, so a
-    march longer than one session runs in pieces and ends where an
-    uninterrupted one would (FP64 exactly).  At the plane, the per-line
+    continues from it, so a march longer than one session runs in pieces (the
+    saved state is what a step reads, the step count included: it seeds the
+    mixed policy's rounding).  At the plane, the per-line
     fluence TSTEP sum_t |E|^2, peak intensity and peak plasma density
     (gs_line_diagnostics) are saved with the field.  prec "fp64_perturbed" is
     the FP64 march from FIELD and BIGBUFFER2 with seeded relative noise 1e-12
@@ -323,6 +323,7 @@ def plane_march(gsa, workloads, work, prec, z_plane, state_dir, resume, time_lim
         p.set_scalar("ZSTEP", float(st["dz"]))
         p.set_scalar("n_Z", 1)
         steps0, rej0 = int(st["steps"]), int(st["rejected"])
+        p.set_scalar("STEP_N", steps0)
     elif prec == "fp64_perturbed":
         rng = np.random.default_rng(4242)
         for buf in (gsa.FIELD, gsa.B2):
@@ -333,10 +334,10 @@ def plane_march(gsa, workloads, work, prec, z_plane, state_dir, resume, time_lim
     while not done and time.perf_counter() - t0 < time_limit:
         done = p.march(os.path.join(work, "plane"), chunk) == 0
         print("[bench] plane march %s: %d steps, z = %.6g m, dz = %.3g m, %.1f s"
-              % (prec, steps0 + p.scalar("STEP_N"), p.scalar("CURRENT_Z"), p.scalar("ZSTEP"),
+              % (prec, p.scalar("STEP_N"), p.scalar("CURRENT_Z"), p.scalar("ZSTEP"),
                  time.perf_counter() - t0), file=sys.stderr, flush=True)
     os.makedirs(state_dir, exist_ok=True)
-    steps, rej = steps0 + int(p.scalar("STEP_N")), rej0 + int(p.scalar("n_reject"))
+    steps, rej = int(p.scalar("STEP_N")), rej0 + int(p.scalar("n_reject"))
     out = {"precision": prec, "z": p.scalar("CURRENT_Z"), "steps": steps, "rejected": rej,
            "wall_s": time.perf_counter() - t0, "reached": bool(done),
            "scan_fallbacks": p.scalar("SCAN_FALLBACKS")}
@@ -703,8 +704,11 @@ def main():
                         "kernel_ms_per_step": {k: full[k]["kernel_ms"] for k in full},
                         "autotuned_shapes": {k: full[k]["autotuned"] for k in full}},
         "autotuned_shapes": {k: res[k]["autotuned"] for k in res},
-        "fp32_long_march_note": "over the 1.06M-step 256x1024 march to 2 cm the FP32 policy (round-to-nearest "
-                                "state) ends 3.9e-2 from FP64, mixed 4.0e-5 (profiles/accuracy_256x1024_full_march.md)",
+        "fp32_long_march_note": "adaptive marches through collapse: at the 1 cm plane of this workload (1.8M steps) "
+                                "the FP32 policy (round-to-nearest state) is 1.2e-1 from FP64 in the field and "
+                                "1.6e-2 in the fluence, mixed 4.2e-5 and 6.6e-6 (profiles/accuracy_1024x4096_1cm.md); "
+                                "over the 1.06M-step 256x1024 march to 2 cm FP32 ends 3.9e-2 from FP64, mixed "
+                                "4.0e-5 (profiles/accuracy_256x1024_full_march.md)",
         "field_rel_l2_vs_fp64": err or None,
         "accuracy_march_vs_fp64": acc,
         "energy_width_scan": batch,

@@ -48,7 +48,9 @@ class PhaseMarch(torch.autograd.Function):
 
     phi: real tensor [N_T]; field0: complex [N_R, N_T] (r, omega) numpy array;
     returns the complex128 field [N_R, N_T] after the march (CPU tensor).  The
-    plan's state is overwritten.  The plan keeps the checkpoints of its last
+    plan's state is overwritten: every forward marches from z = 0 and step
+    count 0 (the step count seeds the mixed policy's rounding), so the same phi
+    gives the same bits on every call.  The plan keeps the checkpoints of its last
     forward only: a backward after a later forward on the same plan raises."""
 
     @staticmethod
@@ -58,6 +60,8 @@ class PhaseMarch(torch.autograd.Function):
         plan.put(A0, FIELD)
         plan.put(A0, B2)
         plan.qdht(B2)
+        plan.set_scalar("STEP_N", 0)
+        plan.set_scalar("CURRENT_Z", 0.0)
         plan.adjoint_forward(dz_rk, dz_lin)
         plan._adjoint_generation = getattr(plan, "_adjoint_generation", 0) + 1
         ctx.plan = plan

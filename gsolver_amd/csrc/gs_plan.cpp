@@ -167,6 +167,7 @@ struct gs_plan {
     double adj_budget = -1;                // bytes of per-step checkpoints kept without recomputation
                                            // (< 0: 60 % of the device memory free at the forward)
     long adj_seed0 = 0;                    // STEP_N at the first recorded step (rounding seeds of the replay)
+    bool adj_dirty0 = false;               // FIELD's skipped columns were non-zero at the first recorded step
     double last_dz_nl = 0;     // NL step size of the last accepted gs_step attempt
 
     void* buf(int id)
@@ -1240,6 +1241,12 @@ int gs_set_scalar(gs_plan* p, const char* name, double v)
     const std::string n(name);
     if (n == "ZSTEP") p->ZSTEP = v;
     else if (n == "CURRENT_Z") p->CURRENT_Z = v;
+    else if (n == "STEP_N") {
+        // step attempts so far; it also seeds the mixed policy's unbiased rounding
+        // of each step, so a march resumed from a saved state sets it back
+        if (!(v >= 0 && v < 9.0e15) || v != (double)(long)v) return fail(GS_E_ARG, "STEP_N must be a non-negative integer");
+        p->STEP_N = (long)v;
+    }
     else if (n == "n_Z") {
         if (!(v >= 0 && v <= p->s.NZ - 1) || v != (double)(int)v)
             return fail(GS_E_ARG, "n_Z must be an integer plane index in 0..N_Z-1");
@@ -1766,6 +1773,10 @@ int gs_adjoint_forward(gs_plan* p, int n, const double* dz_rk, const double* dz_
         }
         const void* A = p->prec == gs::kFP64 ? p->at64.p : p->amat.p;
         gs::LstepArgs ls = p->ls;
+        // the march's transforms (column list, splits and tile shape of gs_step);
+        // the first QDHT back clears FIELD's skipped columns when they are not zero
+        const bool dirty0 = p->dead_dirty;
+        const gs::QCols qf = p->qcols(false, false), qb0 = p->qcols(dirty0, true), qb = p->qcols(false, true);
         for (int k = 0; k < n; ++k) {
             if (K == n) {
                 HCK(hipMemcpyAsync((char*)p->ckpt.p + (size_t)k * fb, p->field.p, fb, hipMemcpyDeviceToDevice, p->st));
@@ -1777,13 +1788,14 @@ int gs_adjoint_forward(gs_plan* p, int n, const double* dz_rk, const double* dz_
             LCK(gs::launch_nl_rk4(p->prec, nt, p->field.p, p->b1.p, dz_rk[k], p->nl, p->linemax.as<double>(), p->st));
             ls.dz = dz_lin[k];
             ls.seed = (uint32_t)(p->STEP_N + k);
-            LCK(gs::launch_qdht(p->prec, rp, nt, A, p->b1.p, nullptr, &ls, p->st));
-            LCK(gs::launch_qdht(p->prec, rp, nt, A, p->fk.p, p->field.p, nullptr, p->st));
+            LCK(gs::launch_qdht(p->prec, rp, nt, A, p->b1.p, nullptr, &ls, p->st, nullptr, &qf));
+            LCK(gs::launch_qdht(p->prec, rp, nt, A, p->fk.p, p->field.p, nullptr, p->st, nullptr, k ? &qb : &qb0));
         }
         check_status(p, nullptr);
         p->adj_rk.assign(dz_rk, dz_rk + n);
         p->adj_seed0 = p->STEP_N;
-        p->dead_dirty = false;     // its QDHTs back transform every column
+        p->adj_dirty0 = dirty0;
+        p->dead_dirty = false;
         p->adj_lin.assign(dz_lin, dz_lin + n);
         for (int k = 0; k < n; ++k) p->CURRENT_Z += dz_lin[k];
         p->ZSTEP = dz_lin[n - 1];
@@ -1808,6 +1820,12 @@ int gs_adjoint_backward(gs_plan* p, const double* gfield, const double* gb2, int
         gs::LstepArgs la = p->ls;
         gs::LstepArgs lr = p->ls;        // the replay's linear step updates the replay buffers
         lr.fk = p->rfk.p;
+        // the forward's transforms for the replay; the transposed ones skip the same
+        // columns: the linear step masks them (its transpose drops their cotangent),
+        // so the cotangents of skipped columns of FIELD_k and B1 are never read and
+        // stay at the zeros set here
+        const gs::QCols qf = p->qcols(false, false), qb0 = p->qcols(p->adj_dirty0, true), qb = p->qcols(false, true);
+        HCK(hipMemsetAsync(p->gT1.p, 0, fb, p->st));
         for (int k1 = n; k1 > 0;) {
             const int k0 = (k1 - 1) / K * K;
             if (K < n) {
@@ -1821,17 +1839,17 @@ int gs_adjoint_backward(gs_plan* p, const double* gfield, const double* gb2, int
                     LCK(gs::launch_nl_rk4(p->prec, nt, p->rfield.p, p->rb1.p, p->adj_rk[k], p->nl, p->linemax.as<double>(), p->st));
                     lr.dz = p->adj_lin[k];
                     lr.seed = (uint32_t)(p->adj_seed0 + k);
-                    LCK(gs::launch_qdht(p->prec, rp, nt, A, p->rb1.p, nullptr, &lr, p->st));
-                    LCK(gs::launch_qdht(p->prec, rp, nt, A, p->rfk.p, p->rfield.p, nullptr, p->st));
+                    LCK(gs::launch_qdht(p->prec, rp, nt, A, p->rb1.p, nullptr, &lr, p->st, nullptr, &qf));
+                    LCK(gs::launch_qdht(p->prec, rp, nt, A, p->rfk.p, p->rfield.p, nullptr, p->st, nullptr, k ? &qb : &qb0));
                 }
             }
             for (int k = k1 - 1; k >= k0; --k) {
                 // FIELD_{k+1} = Q FIELD_k(k-space): cotangent of FIELD_k(k-space)
-                LCK(gs::launch_qdht(pa, rp, nt, p->adjmat.p, p->gA.p, p->gT1.p, nullptr, p->st));
+                LCK(gs::launch_qdht(pa, rp, nt, p->adjmat.p, p->gA.p, p->gT1.p, nullptr, p->st, nullptr, &qb));
                 la.dz = p->adj_lin[k];
                 LCK(gs::launch_lstep_adj(p->prec, rp, nt, p->gT1.p, p->gB.p, p->gT2.p, la, p->st));
                 // B1(k-space) = Q B1
-                LCK(gs::launch_qdht(pa, rp, nt, p->adjmat.p, p->gT2.p, p->gT1.p, nullptr, p->st));
+                LCK(gs::launch_qdht(pa, rp, nt, p->adjmat.p, p->gT2.p, p->gT1.p, nullptr, p->st, nullptr, &qf));
                 LCK(gs::launch_nl_rk4_vjp(p->prec, nt, (const char*)p->ckpt.p + (size_t)(k - k0) * fb, p->gT1.p, p->gA.p,
                                           p->stage.p, (size_t)rp * nt, p->adj_rk[k], p->nl, p->st));
             }

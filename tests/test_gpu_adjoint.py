@@ -74,6 +74,26 @@ def test_adjoint_segmented_matches_single_segment(tmp_path, prec):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+def test_adjoint_forward_is_the_march(tmp_path, prec):
+    """gs_adjoint_forward replays recorded steps with the march's own
+    transforms (column skipping, splits, tile shapes, rounding seeds): from
+    the same initial state it ends on the FIELD and BIGBUFFER2 of the
+    controller's march bit for bit (N_R = 256)."""
+    from gsolver_amd.adjoint import record_steps
+    cfg = write_cfg(tmp_path / "c.txt", **dict(HOT, N_R="256"))
+    rec = gsa.Plan(cfg, precision=prec)
+    assert rec.scalar("QDHT_SKIP_COLUMNS") == 1
+    dz_rk, dz_lin = record_steps(rec, n_steps=6)
+    f_rec, b_rec = rec.get(), rec.get(gsa.B2)
+    rec.close()
+    p = gsa.Plan(cfg, precision=prec)
+    p.adjoint_forward(dz_rk, dz_lin)
+    assert np.array_equal(p.get(), f_rec)
+    assert np.array_equal(p.get(gsa.B2), b_rec)
+    p.close()
+
+
 @pytest.mark.parametrize("case", ["no_raman", "plasma_only", "kerr_only"])
 def test_adjoint_physics_variants_vs_oracle(tmp_path, case):
     """The transposed march with parts of the RHS switched off (FP64, 3 steps):
@@ -184,3 +204,29 @@ def test_phase_march_autograd(tmp_path):
         worst = max(worst, abs(g @ v - d) / abs(d))
     print("phase", worst)
     assert worst < 1e-6
+
+
+def test_phase_march_repeatable_mixed(tmp_path):
+    """Mixed precision: every PhaseMarch forward starts from step count 0 (the
+    seed of the unbiased rounding), so a second gradient of the same phi, and
+    one with sqrt-segmented checkpoints, repeat the first bit for bit
+    (N_R = 256)."""
+    torch = pytest.importorskip("torch")
+    from gsolver_amd.adjoint import PhaseMarch, record_steps
+    cfg = write_cfg(tmp_path / "c.txt", **dict(HOT, N_R="256"))
+    rec = gsa.Plan(cfg, precision="mixed")
+    field0 = rec.get(gsa.FIELD)
+    dz_rk, dz_lin = record_steps(rec, n_steps=9)
+    rec.close()
+    p = gsa.Plan(cfg, precision="mixed")
+    rng = np.random.default_rng(3)
+    phi0 = 0.1 * rng.standard_normal(p.T)
+    grads = []
+    for budget in (-1.0, -1.0, 1.0):
+        p.set_scalar("ADJOINT_CHECKPOINT_BYTES", budget)
+        phi = torch.tensor(phi0, requires_grad=True)
+        (torch.fft.ifft(PhaseMarch.apply(phi, p, field0, dz_rk, dz_lin)[0]).abs() ** 4).sum().backward()
+        grads.append(phi.grad.numpy().copy())
+    assert p.scalar("ADJOINT_SEGMENT") < 9
+    assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
+    p.close()

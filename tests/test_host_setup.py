@@ -107,6 +107,9 @@ def test_phase_march_backward_after_later_forward_raises():
         def qdht(self, buf):
             pass
 
+        def set_scalar(self, name, v):
+            pass
+
         def adjoint_forward(self, rk, lin):
             pass
 
