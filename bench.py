@@ -43,9 +43,9 @@ def run_precision(gsa, cfg, prec, steps, warmup, dz, skip=True):
     live_cols = 32 * p.scalar("QDHT_LIVE_BLOCKS") if skip else p.T
     tiles = {"nl_samples_per_thread": int(p.scalar("NL_SAMPLES_PER_THREAD")),
              "qdht_xcd_grouping": "by row tile" if p.scalar("QDHT_TILE_ORDER") else "by column tile"}
-    if prec == "mixed":
+    if prec in ("mixed", "fp32"):
         tiles["qdht_k_groups"] = int(p.scalar("QDHT_K_GROUPS"))
-    elif prec == "fp32" and not skip:
+    if prec == "fp32" and not skip:
         tiles["qdht_narrow_tiles"] = bool(p.scalar("QDHT_F32_NARROW_FULL"))
     print("[bench] %s: autotuned shapes %s" % (prec, tiles), file=sys.stderr, flush=True)
     if warmup > 0:

@@ -59,7 +59,7 @@ struct QCols {
     int nzt;
     int zero_dead;      // host-driven launches: clear the skipped blocks of the output
     int split3;         // mixed: operands in three bf16 parts, six products (FP32-level accuracy)
-    int ks;             // mixed: K groups per narrow-tile workgroup, 1 or 2 (same bits; 0: 2)
+    int ks;             // mixed / FP32: K groups per workgroup, 1 or 2 (same bits; 0: the default)
     int order;          // tiles per XCD: 0 all row tiles of a column tile, 1 all column tiles of a row tile
     int wide;           // FP32 all-column transform: 64-column tiles (else the 32-column ones; same bits)
 };

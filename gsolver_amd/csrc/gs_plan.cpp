@@ -598,8 +598,8 @@ void autotune_nl(gs_plan* p)
 // a (precision, N_R, N_T) in the process, separately for the column-skipping
 // and the all-column transform (back, the one without the linear step):
 // the tile order over the XCDs (all row tiles of a column tile per XCD, or all
-// column tiles of a row tile) in every precision; mixed, one or two K groups
-// per narrow-tile workgroup (256 or 512 threads); FP32 all-column, 64- or
+// column tiles of a row tile) in every precision; mixed and FP32, one or two
+// K groups per workgroup (256 or 512 threads); FP32 all-column, 64- or
 // 32-column tiles.  The default (the first candidate) is replaced only by a
 // clear (3 %) win.  GS_QDHT_KSPLIT fixes the mixed K groups for the process.
 struct QShape {
@@ -637,10 +637,11 @@ void autotune_qdht(gs_plan* p)
         if (mode == 0 && p->nct == 0) continue;
         std::vector<QShape> cand;
         for (int order : {0, 1}) {
-            if (p->prec == gs::kMIXED) {
+            if (p->prec == gs::kMIXED || (p->prec == gs::kFP32 && mode == 0)) {
                 for (int ks : {2, 1}) cand.push_back(QShape{ks, order, false});
-            } else if (p->prec == gs::kFP32 && mode == 1) {
-                for (bool nar : {false, true}) cand.push_back(QShape{2, order, nar});
+            } else if (p->prec == gs::kFP32) {
+                for (bool nar : {false, true})
+                    for (int ks : {2, 1}) cand.push_back(QShape{ks, order, nar});
             } else {
                 cand.push_back(QShape{2, order, false});
             }
@@ -1266,7 +1267,7 @@ int gs_set_scalar(gs_plan* p, const char* name, double v)
     }
     else if (n == "QDHT_K_GROUPS" || n == "QDHT_TILE_ORDER" || n == "QDHT_F32_NARROW_FULL") {
         // the march's GEMM tile shape (normally autotuned; the same bits either way):
-        // K groups of the mixed narrow tiles in the current column mode (1 or 2),
+        // K groups of the mixed / FP32 tiles in the current column mode (1 or 2),
         // the tile order over the XCDs in the current column mode (0 or 1), or
         // FP32 all-column transforms on the narrow tiles (0 or 1)
         const bool groups = n == "QDHT_K_GROUPS";

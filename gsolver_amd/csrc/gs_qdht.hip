@@ -67,22 +67,34 @@ __device__ __forceinline__ void zero_skipped(T* __restrict__ Y, int NC, int mt, 
 // 32 rows x 32 columns (BNC = 32, the column-skipping transform: ~900 live
 // columns at N_T = 4096 still give 29 x 8 tiles).  Each output sums its k
 // terms in the same order at either width, so the two agree bit for bit.
-template <bool LSTEP, int BNC>
+// FP32: 128 rows x BNC complex columns per group of 4 waves, K step 16, A and B
+// staged in LDS.  As in the split-BF16 narrow tiles every output is (sum over
+// the first K/2 rows) + (sum over the last K/2 rows), each half in k order:
+// KS = 1 group sums both halves in turn, KS = 2 groups (512 threads) one half
+// each, added through LDS, each group writing half of the rows.  Same bits
+// for every (BNC, KS).  sm holds smem_f32<BNC, KS>() floats.
+template <int BNC, int KS>
+constexpr int smem_f32()
+{
+    constexpr int tiles = KS * 16 * ((128 + 32) + 2 * (BNC + 32));                 // As, Br, Bi per group
+    constexpr int xch = (KS - 1) * 2 * (BNC == 64 ? 2 : 1) * 2 * 8 * 256;          // half the sums, both ways
+    return tiles > xch ? tiles : xch;
+}
+template <bool LSTEP, int BNC, int KS = 1>
 __device__ __forceinline__ void qdht_f32_body(int bid, const float* __restrict__ At, const C2<float>* __restrict__ X,
-                                              C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, const QCols& qc)
+                                              C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, const QCols& qc,
+                                              float* __restrict__ sm)
 {
     constexpr int BM = 128, BK = 16, PA = 32, PB = 32;
     constexpr int RB = BNC == 64 ? 2 : 1;      // 32-row blocks per wave
     static_assert(BNC == 64 || BNC == 32, "tile width");
+    static_assert(KS == 1 || KS == 2, "one or two K groups");
     const DevCtl* ctl = la.ctl;
     if (ctl) {
         if (ctl->state != 0 || !ctl->accept) return;
         la.dz = ctl->dz_lin;
         la.seed = (uint32_t)ctl->steps;
     }
-    __shared__ float As[BK][BM + PA];
-    __shared__ float Br[BK][BNC + PB];
-    __shared__ float Bi[BK][BNC + PB];
     const int M = K;
     const int mt = M / BM, nct = qc.ctile ? qc.nct : NC / BNC;
     if (bid >= mt * nct) {
@@ -92,11 +104,17 @@ __device__ __forceinline__ void qdht_f32_body(int bid, const float* __restrict__
     int mi, ni;
     tile_of(bid, mt, nct, &mi, &ni, qc.order != 0);
     const int m0 = mi * BM, n0 = col0_of(qc, ni, BNC);
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x & 255, lane = tid & 63, w = tid >> 6;
+    const int ks = KS == 1 ? 0 : (int)(threadIdx.x >> 8);
     const int wm = BNC == 64 ? (w >> 1) * 64 : w * 32, wn = BNC == 64 ? (w & 1) * 32 : 0;
     const int lr = tid >> 4, lc = (tid & 15);
+    typedef float TA[BM + PA];
+    typedef float TB[BNC + PB];
+    TA* As = reinterpret_cast<TA*>(sm + ks * 16 * ((BM + PA) + 2 * (BNC + PB)));
+    TB* Br = reinterpret_cast<TB*>(reinterpret_cast<float*>(As) + BK * (BM + PA));
+    TB* Bi = reinterpret_cast<TB*>(reinterpret_cast<float*>(Br) + BK * (BNC + PB));
 
-    v16f acc[RB][2];
+    v16f acc[RB][2], first[RB][2];
 #pragma unroll
     for (int a = 0; a < RB; ++a)
 #pragma unroll
@@ -124,11 +142,24 @@ __device__ __forceinline__ void qdht_f32_body(int bid, const float* __restrict__
             Br[lr][c + 1] = rb[j].z; Bi[lr][c + 1] = rb[j].w;
         }
     };
-    gload(0);
+    const int kh = K / 2, kbeg = ks * (K / KS), kend = kbeg + K / KS;
+    gload(kbeg);
     sstore();
     __syncthreads();
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        const bool more = k0 + BK < K;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        if constexpr (KS == 1) {
+            if (k0 == kh) {
+#pragma unroll
+                for (int a = 0; a < RB; ++a)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        first[a][c] = acc[a][c];
+#pragma unroll
+                        for (int r = 0; r < 16; ++r) acc[a][c][r] = 0.f;
+                    }
+            }
+        }
+        const bool more = k0 + BK < kend;
         if (more) gload(k0 + BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 2) {
@@ -147,23 +178,70 @@ __device__ __forceinline__ void qdht_f32_body(int bid, const float* __restrict__
             __syncthreads();
         }
     }
+    if constexpr (KS == 1) {
+#pragma unroll
+        for (int a = 0; a < RB; ++a)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[a][c][r] = first[a][c][r] + acc[a][c][r];
+    }
+    if constexpr (KS == 2) {
+        // group g keeps rows r >> 3 == g of each 32-row block; the other half of its
+        // sums goes to the partner through LDS (lane-contiguous), then each adds the
+        // partner's: y = first half + second half in both groups
+        constexpr int NX = RB * 2 * 8;     // floats handed over per lane
+        if (ks == 0) {
+#pragma unroll
+            for (int a = 0; a < RB; ++a)
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) sm[((NX + (a * 2 + c) * 8 + j) * 256) + tid] = acc[a][c][8 + j];
+        } else {
+#pragma unroll
+            for (int a = 0; a < RB; ++a)
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) sm[(((a * 2 + c) * 8 + j) * 256) + tid] = acc[a][c][j];
+        }
+        __syncthreads();
+        if (ks == 0) {
+#pragma unroll
+            for (int a = 0; a < RB; ++a)
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) acc[a][c][j] += sm[(((a * 2 + c) * 8 + j) * 256) + tid];
+        } else {
+#pragma unroll
+            for (int a = 0; a < RB; ++a)
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) acc[a][c][8 + j] += sm[((NX + (a * 2 + c) * 8 + j) * 256) + tid];
+        }
+    }
     const int col = n0 + wn + (lane & 31);
     const float amp = LSTEP ? lstep_amp<PolF32>(col, la) : 1.f;
 #pragma unroll
     for (int rb2 = 0; rb2 < RB; ++rb2)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
+            if (KS == 2 && (r >> 3) != ks) continue;
             const int row = m0 + wm + rb2 * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             const size_t o = (size_t)row * NC + col;
             if (LSTEP) lstep_point<PolF32>(o, acc[rb2][0][r], acc[rb2][1][r], amp, la);
             else Y[o] = C2<float>{acc[rb2][0][r], acc[rb2][1][r]};
         }
 }
-template <bool LSTEP, int BNC>
-__global__ __launch_bounds__(256) void k_qdht_f32(const float* __restrict__ At, const C2<float>* __restrict__ X,
-                                                  C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, QCols qc)
+template <bool LSTEP, int BNC, int KS>
+__global__ __launch_bounds__(256 * KS) void k_qdht_f32(const float* __restrict__ At, const C2<float>* __restrict__ X,
+                                                       C2<float>* __restrict__ Y, int K, int NC, LstepArgs la, QCols qc)
 {
-    qdht_f32_body<LSTEP, BNC>(blockIdx.x, At, X, Y, K, NC, la, qc);
+    __shared__ __align__(16) float sm[smem_f32<BNC, KS>()];
+    qdht_f32_body<LSTEP, BNC, KS>(blockIdx.x, At, X, Y, K, NC, la, qc, sm);
 }
 
 // BF16 3-pass: same tile, K step 32, operands k-contiguous in LDS (split2 in gs_dev.h).
@@ -841,13 +919,14 @@ __global__ __launch_bounds__(256 * KS) void k_qdht_t32_batch(const GemmRun* __re
     if (g.qc.split3) qdht_t32_body<LSTEP, 3, KS>(bid, (const uint16_t*)g.A, (const C2<float>*)g.X, (C2<float>*)g.Y, g.K, g.NC, g.la, g.qc, sm);
     else qdht_t32_body<LSTEP, 2, KS>(bid, (const uint16_t*)g.A, (const C2<float>*)g.X, (C2<float>*)g.Y, g.K, g.NC, g.la, g.qc, sm);
 }
-template <bool LSTEP>
-__global__ __launch_bounds__(256) void k_qdht_f32_batch(const GemmRun* __restrict__ runs, const int* __restrict__ brun)
+template <bool LSTEP, int KS>
+__global__ __launch_bounds__(256 * KS) void k_qdht_f32_batch(const GemmRun* __restrict__ runs, const int* __restrict__ brun)
 {
+    __shared__ __align__(16) float sm[smem_f32<32, KS>()];
     const GemmRun& g = runs[brun[blockIdx.x]];
     const int bid = (int)blockIdx.x - g.b0;
     if (bid >= g.nb) return;
-    qdht_f32_body<LSTEP, 32>(bid, (const float*)g.A, (const C2<float>*)g.X, (C2<float>*)g.Y, g.K, g.NC, g.la, g.qc);
+    qdht_f32_body<LSTEP, 32, KS>(bid, (const float*)g.A, (const C2<float>*)g.X, (C2<float>*)g.Y, g.K, g.NC, g.la, g.qc, sm);
 }
 template <bool LSTEP>
 __global__ __launch_bounds__(256) void k_qdht_f64_batch(const GemmRun* __restrict__ runs, const int* __restrict__ brun)
@@ -893,8 +972,14 @@ int launch_qdht_batch(Prec p, bool lstep, const GemmRun* runs, const int* brun, 
         if (lstep) hipLaunchKernelGGL(k_qdht_f64_batch<true>, grid, blk, 0, st, runs, brun);
         else hipLaunchKernelGGL(k_qdht_f64_batch<false>, grid, blk, 0, st, runs, brun);
     } else if (p == kFP32) {
-        if (lstep) hipLaunchKernelGGL(k_qdht_f32_batch<true>, grid, blk, 0, st, runs, brun);
-        else hipLaunchKernelGGL(k_qdht_f32_batch<false>, grid, blk, 0, st, runs, brun);
+        if (ks == 1) {
+            if (lstep) hipLaunchKernelGGL((k_qdht_f32_batch<true, 1>), grid, blk, 0, st, runs, brun);
+            else hipLaunchKernelGGL((k_qdht_f32_batch<false, 1>), grid, blk, 0, st, runs, brun);
+        } else {
+            const dim3 b2(512);
+            if (lstep) hipLaunchKernelGGL((k_qdht_f32_batch<true, 2>), grid, b2, 0, st, runs, brun);
+            else hipLaunchKernelGGL((k_qdht_f32_batch<false, 2>), grid, b2, 0, st, runs, brun);
+        }
     } else if (t32_groups(ks) == 2) {
         const dim3 b2(512);
         if (lstep) hipLaunchKernelGGL((k_qdht_t32_batch<true, 2>), grid, b2, 0, st, runs, brun);
@@ -992,12 +1077,16 @@ int launch_qdht(Prec p, int rp, int nt, const void* amat, const void* in, void* 
         const int mt = rp / 128;
         if (cols && (qc.ctile || !qc.wide)) {
             const dim3 grid(mt * ncb + zblocks(mt));
-            if (ls) hipLaunchKernelGGL((k_qdht_f32<true, 32>), grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
-            else hipLaunchKernelGGL((k_qdht_f32<false, 32>), grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
+            const bool two = qc.ks != 1;
+            auto kt = two ? k_qdht_f32<true, 32, 2> : k_qdht_f32<true, 32, 1>;
+            auto kf = two ? k_qdht_f32<false, 32, 2> : k_qdht_f32<false, 32, 1>;
+            hipLaunchKernelGGL(ls ? kt : kf, grid, dim3(two ? 512 : 256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
         } else {
             const dim3 grid(mt * (nt / 64));
-            if (ls) hipLaunchKernelGGL((k_qdht_f32<true, 64>), grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
-            else hipLaunchKernelGGL((k_qdht_f32<false, 64>), grid, dim3(256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
+            const bool two = !cols || qc.ks != 1;
+            auto kt = two ? k_qdht_f32<true, 64, 2> : k_qdht_f32<true, 64, 1>;
+            auto kf = two ? k_qdht_f32<false, 64, 2> : k_qdht_f32<false, 64, 1>;
+            hipLaunchKernelGGL(ls ? kt : kf, grid, dim3(two ? 512 : 256), 0, st, (const float*)amat, (const C2<float>*)in, (C2<float>*)out, rp, nt, la, qc);
         }
     } else {
         if (rp % 128 || nt % 128) return -1001;

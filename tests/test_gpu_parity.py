@@ -528,9 +528,9 @@ def test_column_skipping_equals_full_transforms(tmp_path, prec, nr):
 def test_qdht_tile_shapes_give_identical_bits(tmp_path, prec):
     """The GEMM tile autotuner chooses among tile shapes that give the same
     bits: in every precision the tile order over the XCDs (QDHT_TILE_ORDER);
-    mixed, the narrow split-BF16 tiles with one or two K groups per workgroup
-    (each output = first-half sum + second-half sum either way); FP32, 64- or
-    32-column tiles for all-column transforms.  Three fixed steps and a short
+    mixed and FP32, one or two K groups per workgroup (each output =
+    first-half sum + second-half sum either way); FP32, 64- or 32-column tiles
+    for all-column transforms.  Three fixed steps and a short
     adaptive march leave FIELD and BIGBUFFER2 identical for every choice, in
     the column-skipping and the all-column march (N_R = 256: both K halves
     hold operator rows)."""
@@ -539,7 +539,9 @@ def test_qdht_tile_shapes_give_identical_bits(tmp_path, prec):
     if prec == "mixed":
         cases += [(skip, {"QDHT_K_GROUPS": g, "QDHT_TILE_ORDER": 1}) for skip in (1, 0) for g in (1, 2)]
     elif prec == "fp32":
-        cases += [(0, {"QDHT_F32_NARROW_FULL": v, "QDHT_TILE_ORDER": 1}) for v in (0, 1)]
+        cases += [(0, {"QDHT_F32_NARROW_FULL": v, "QDHT_K_GROUPS": g, "QDHT_TILE_ORDER": 1})
+                  for v in (0, 1) for g in (1, 2)]
+        cases += [(1, {"QDHT_K_GROUPS": g}) for g in (1, 2)]
     got = []
     for skip, sets in cases:
         p = gsa.Plan(cfg, precision=prec)
