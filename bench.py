@@ -281,13 +281,18 @@ def batch_check(gsa, workloads, work, nruns, z_max, seed=20261202):
     gsa.march_many(plans, [d] * nruns)
     t_batch = time.perf_counter() - t0
     diff = max(float(np.abs(p.get() - f).max()) for p, f in zip(plans, single))
+    kernels = int(plans[0].scalar("BATCH_GRAPH_KERNELS"))
     for p in plans:
         p.close()
     out = {"runs": nruns, "z_max_m": z_max, "accepted_steps": steps,
+           "kernels_per_batched_attempt": kernels,
+           "kernels_per_single_attempt": 5,
            "batched_steps_per_s": steps / t_batch, "single_steps_per_s": steps / t_single,
            "speedup": t_single / t_batch, "max_abs_field_diff_batched_vs_single": diff}
-    print("[bench] batch: %d runs, %d steps, batched %.0f steps/s, one by one %.0f steps/s, max diff %g"
-          % (nruns, steps, out["batched_steps_per_s"], out["single_steps_per_s"], diff), file=sys.stderr, flush=True)
+    print("[bench] batch: %d runs, %d steps, batched %.0f steps/s (%d kernels per attempt of all runs), "
+          "one by one %.0f steps/s, max diff %g"
+          % (nruns, steps, out["batched_steps_per_s"], kernels, out["single_steps_per_s"], diff),
+          file=sys.stderr, flush=True)
     return out
 
 
@@ -304,7 +309,8 @@ def plane_march(gsa, workloads, work, prec, z_plane, state_dir, resume, time_lim
     fluence TSTEP sum_t |E|^2, peak intensity and peak plasma density
     (gs_line_diagnostics) are saved with the field.  prec "fp64_perturbed" is
     the FP64 march from FIELD and BIGBUFFER2 with seeded relative noise 1e-12
-    (how far a rounding-level perturbation grows over the same march)."""
+    added at its start (z = 0, or the resumed state unless that state is
+    itself a perturbed one): how far a rounding-level perturbation grows."""
     import numpy as np
     cfg = workloads.write_config(os.path.join(work, "plane"), "c1", N_Z=2, Z_MAX=z_plane)
     p = gsa.Plan(cfg, precision="fp64" if prec == "fp64_perturbed" else prec)
@@ -324,7 +330,7 @@ def plane_march(gsa, workloads, work, prec, z_plane, state_dir, resume, time_lim
         p.set_scalar("n_Z", 1)
         steps0, rej0 = int(st["steps"]), int(st["rejected"])
         p.set_scalar("STEP_N", steps0)
-    elif prec == "fp64_perturbed":
+    if prec == "fp64_perturbed" and not (resume and "perturbed" in os.path.basename(resume)):
         rng = np.random.default_rng(4242)
         for buf in (gsa.FIELD, gsa.B2):
             f = p.get(buf)

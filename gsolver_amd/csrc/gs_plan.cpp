@@ -168,6 +168,7 @@ struct gs_plan {
                                            // (< 0: 60 % of the device memory free at the forward)
     long adj_seed0 = 0;                    // STEP_N at the first recorded step (rounding seeds of the replay)
     bool adj_dirty0 = false;               // FIELD's skipped columns were non-zero at the first recorded step
+    double batch_nodes = 0;                // kernels per step attempt of the last gs_march_many over this plan
     double last_dz_nl = 0;     // NL step size of the last accepted gs_step attempt
 
     void* buf(int id)
@@ -1023,6 +1024,7 @@ struct BatchMarch {
     std::vector<NLGroup> nlg;
     std::vector<GemmLaunch> gl;
     hipGraphExec_t exec = nullptr;
+    size_t graph_nodes = 0;         // kernels in the captured attempt of all n runs
 
     BatchMarch(gs_plan* const* plans, int n_) : n(n_)
     {
@@ -1100,6 +1102,7 @@ struct BatchMarch {
             throw;
         }
         HCK(hipStreamEndCapture(st, &graph));
+        (void)hipGraphGetNodes(graph, nullptr, &graph_nodes);
         const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         HCK(e);
@@ -1222,6 +1225,7 @@ static int get_scalar(gs_plan* p, const char* name, double* v)
     else if (n == "QDHT_SKIP_COLUMNS") *v = p->qskip ? 1 : 0;
     else if (n == "QDHT_K_GROUPS") *v = p->qks[p->qskip ? 0 : 1];
     else if (n == "QDHT_TILE_ORDER") *v = p->qorder[p->qskip ? 0 : 1];
+    else if (n == "BATCH_GRAPH_KERNELS") *v = p->batch_nodes;
     else if (n == "QDHT_F32_NARROW_FULL") *v = p->f32_narrow_full ? 1 : 0;
     else if (n == "SCAN_FALLBACKS") {
         // line launches whose plasma scan needed the chunk-sequential fallback, since plan creation
@@ -1573,6 +1577,7 @@ int gs_march_many(gs_plan* const* plans, int n, const char* const* outdirs, int*
                 if (plans[j] == plans[i]) return fail(GS_E_ARG, "gs_march_many: a plan is listed twice");
         }
         BatchMarch b(plans, n);
+        for (int i = 0; i < n; ++i) plans[i]->batch_nodes = (double)b.graph_nodes;
         std::vector<DumpSet> ds(n);
         std::vector<char> active(n, 1);
         int left = n;
