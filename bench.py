@@ -548,7 +548,7 @@ def main():
                     help="only march the production workload to --plane-z in this precision, saving its state "
                          "and, at the plane, its field and line diagnostics under --state-dir")
     ap.add_argument("--plane-z", type=float, default=1e-2)
-    ap.add_argument("--state-dir", default="gpurun_out/plane")
+    ap.add_argument("--state-dir", default="plane_state")
     ap.add_argument("--resume", default=None, metavar="STATE_NPZ",
                     help="continue the --plane-march from a state file an earlier --plane-march wrote")
     ap.add_argument("--time-limit", type=float, default=900.0, help="seconds of --plane-march before it saves")
