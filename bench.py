@@ -211,10 +211,9 @@ def traffic_profile_current():
         return False
 
 
-def t32_kernel(products, lstep):
+def t32_kernel(products, lstep, ks):
     """rocprofv3 name of the narrow split-BF16 QDHT kernel the mixed march runs
-    (two or three bf16 parts: 3 or 6 products; K groups from GS_QDHT_KSPLIT)."""
-    ks = 1 if os.environ.get("GS_QDHT_KSPLIT") == "1" else 2
+    (two or three bf16 parts: 3 or 6 products; ks K groups per workgroup)."""
     return "k_qdht_bf16x%d_t32<%s, %d>" % (products, "true" if lstep else "false", ks)
 
 
@@ -665,7 +664,8 @@ def main():
     cols = m["qdht_columns"]
     qdht_ms = 0.5 * (km["qdht_lstep"] + km["qdht_back"])
     dominant = "nl" if km["nl_rk4"] >= km["qdht_lstep"] + km["qdht_back"] else "qdht"
-    roof = (roofline("qdht", qdht_ms, R, T, "mixed", t32_kernel(6, False), cols) if dominant == "qdht"
+    ks_mix = res["mixed"]["autotuned"]["qdht_k_groups"]
+    roof = (roofline("qdht", qdht_ms, R, T, "mixed", t32_kernel(6, False, ks_mix), cols) if dominant == "qdht"
             else roofline("nl", km["nl_rk4"], R, T, "mixed", "k_nl_rk4<PolMix>"))
 
     acc = None
@@ -721,12 +721,14 @@ def main():
         "phase_gradient": grad,
         "kernel_ms_per_step": {p: res[p]["kernel_ms"] for p in res},
         "roofline": roof,
-        "roofline_qdht": roofline("qdht", km["qdht_back"], R, T, "mixed", t32_kernel(6, False), cols),
-        "roofline_qdht_lstep": roofline("qdht", km["qdht_lstep"], R, T, "mixed", t32_kernel(3, True), cols),
+        "roofline_qdht": roofline("qdht", km["qdht_back"], R, T, "mixed", t32_kernel(6, False, ks_mix), cols),
+        "roofline_qdht_lstep": roofline("qdht", km["qdht_lstep"], R, T, "mixed", t32_kernel(3, True, ks_mix), cols),
         "roofline_qdht_fp32": roofline("qdht", res["fp32"]["kernel_ms"]["qdht_back"], R, T, "fp32",
-                                       "k_qdht_f32<false, 32>", res["fp32"]["qdht_columns"]),
+                                       "k_qdht_f32<false, 32, %d>" % res["fp32"]["autotuned"]["qdht_k_groups"],
+                                       res["fp32"]["qdht_columns"]),
         "roofline_qdht_all_columns": roofline("qdht", full["mixed"]["kernel_ms"]["qdht_back"], R, T, "mixed",
-                                              t32_kernel(6, False)),
+                                              t32_kernel(6, False, full["mixed"]["autotuned"]["qdht_k_groups"])
+                                              + " @all-columns"),
         "traffic_profile_current": traffic_profile_current(),
         "roofline_nl": roofline("nl", km["nl_rk4"], R, T, "mixed", "k_nl_rk4<PolMix>"),
         "roofline_nl_valu": valu_roofline("k_nl_rk4<PolMix>", km["nl_rk4"]),
