@@ -7,11 +7,18 @@
 //  tile.  Complex columns are split into re/im planes when staged into LDS,
 //  so one lane ends up owning both parts of an output and can apply the
 //  linear step in registers.
-//   FP32   v_mfma_f32_32x32x2_f32, tile 128 x 64 complex, K step 16
-//   MIXED  split-BF16 3-pass (hi*hi + hi*lo + lo*hi, FP32 accumulate),
-//          v_mfma_f32_32x32x16_bf16, tile 128 x 128 complex, K step 32,
-//          software-pipelined over double-buffered LDS
+//   FP32   v_mfma_f32_32x32x2_f32, tiles 128 x 64 or 128 x 32 complex, K step
+//          16, one or two K groups per workgroup
+//   MIXED  split-BF16 (FP32 accumulate) on v_mfma_f32_32x32x16_bf16: the
+//          march's transforms on narrow 128 x 32 tiles ("t32", two bf16 parts
+//          and 3 products into the linear step, three parts and 6 products
+//          back, one or two K groups), full transforms of the API on
+//          128 x 128 tiles ("t128"); K step 32, software-pipelined over
+//          double-buffered LDS
 //   FP64   v_mfma_f64_16x16x4_f64, tile 64 x 32 complex, K step 16
+//  The march's transforms can skip the 32-column blocks the linear step
+//  zeroes (QCols); tile shapes that give the same bits (K groups, tile
+//  width, tile order over the XCDs) are chosen by the plan's autotuner.
 #include "gs_dev.h"
 
 #include <cstdlib>
