@@ -84,7 +84,10 @@ int gs_setup_table(const char* cfg_path, int nover, const char* const* over, con
 /* sizes: N_R, N_T, N_Z */
 int gs_dims(gs_plan* p, int* nr, int* nt, int* nz);
 /* named scalars of the plan (ZSTEP, CURRENT_Z, OMEGA0, TSTEP, MAX_TOLERANCE,
- * STEP_N, n_Z, n_reject, GROUP_VELOCITY, OMEGA_MIN, OMEGA_MAX, ...) */
+ * STEP_N, n_Z, n_reject, GROUP_VELOCITY, OMEGA_MIN, OMEGA_MAX, ...).  Settable:
+ * ZSTEP, CURRENT_Z, STEP_N, n_Z, MAX_TOLERANCE, ADJOINT_CHECKPOINT_BYTES and
+ * the GEMM switches QDHT_SKIP_COLUMNS, QDHT_K_GROUPS, QDHT_TILE_ORDER,
+ * QDHT_F32_NARROW_FULL (these change no result). */
 int gs_get_scalar(gs_plan* p, const char* name, double* v);
 int gs_set_scalar(gs_plan* p, const char* name, double v);
 /* named host-side tables built at set-up, FP64: "omega" (T), "wavenum" (T cplx),
