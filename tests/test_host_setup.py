@@ -81,6 +81,22 @@ def test_bench_refuses_more_than_one_gpu():
     assert "one GPU" in r.stderr
 
 
+def test_bench_roofline_kernels_are_in_the_committed_profile():
+    """The kernel names bench.py attaches HBM traffic and VALU counts to are
+    the rocprofv3 names of the committed profile (narrow split-BF16 and FP32
+    tiles with their K groups, the line kernel), so `traffic` is filled."""
+    import importlib
+    import sys
+    sys.path.insert(0, ROOT)
+    bench = importlib.import_module("bench")
+    names = [bench.t32_kernel(6, False, 2), bench.t32_kernel(3, True, 2), "k_qdht_f32<false, 32, 2>",
+             "k_nl_rk4<PolMix>", bench.t32_kernel(6, False, 2) + " @all-columns"]
+    for n in names:
+        t = bench.measured_traffic(n)
+        assert t is not None and t > 0, n
+    assert bench.measured_valu_insts("k_nl_rk4<PolMix>") > 0
+
+
 def test_record_steps_needs_exactly_one_stop():
     """record_steps stops either after n_steps or at z_end: giving both or
     neither is an error before the plan is touched."""
