@@ -267,7 +267,7 @@ def batch_check(gsa, workloads, work, nruns, z_max, seed=20261202):
     d = os.path.join(work, "batch")
     cfgs = workloads.batch_configs(d, nruns, z_max, seed)
     single, t_single, steps = [], 0.0, 0
-    for c in cfgs:
+    for i, c in enumerate(cfgs):
         p = gsa.Plan(c, precision="mixed")
         t0 = time.perf_counter()
         p.march(d)
@@ -275,7 +275,11 @@ def batch_check(gsa, workloads, work, nruns, z_max, seed=20261202):
         single.append(p.get())
         steps += int(p.scalar("STEP_N") - p.scalar("n_reject"))
         p.close()
+        if (i + 1) % 8 == 0:
+            print("[bench] batch: %d of %d runs marched one by one (%d steps, %.1f s)" % (i + 1, nruns, steps, t_single),
+                  file=sys.stderr, flush=True)
     plans = [gsa.Plan(c, precision="mixed") for c in cfgs]
+    print("[bench] batch: marching the %d runs together" % nruns, file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     gsa.march_many(plans, [d] * nruns)
     t_batch = time.perf_counter() - t0
