@@ -94,3 +94,28 @@ def test_sellmeier_table_against_reference_sio2_rawdisp(tmp_path):
     assert d.max() < 5e-3
     near = m & (lam > 0.7) & (lam < 0.9)
     assert np.abs(n.real[near] - W.sellmeier_n(lam[near])).max() < 1e-4
+
+
+def test_batch_configs_draw_the_scan(tmp_path):
+    """The 64-run energy/width scan: energies log-uniform in 0.5-10 uJ, w0 in
+    40-150 um (written as DIAMETER_FWHM = w0 sqrt(2 ln 2)), grids from the two
+    sets, seeded (the same seed gives the same files), each to z_max with
+    planes at 0 and z_max; all three grid sizes of each axis occur."""
+    import math
+    from gsolver_amd import workloads
+    a = workloads.batch_configs(str(tmp_path / "a"), 64, 2e-4)
+    b = workloads.batch_configs(str(tmp_path / "b"), 64, 2e-4)
+    assert len(a) == 64
+    nrs, nts, es, ws = set(), set(), [], []
+    for fa, fb in zip(a, b):
+        ta, tb = open(fa).read(), open(fb).read()
+        assert ta == tb
+        kv = dict(l.split(None, 1) for l in ta.splitlines() if l.strip() and not l.startswith("%"))
+        nrs.add(int(kv["N_R"]))
+        nts.add(int(kv["N_T"]))
+        es.append(float(kv["ENERGY"]))
+        ws.append(float(kv["DIAMETER_FWHM"]) / math.sqrt(2 * math.log(2)))
+        assert int(kv["N_Z"]) == 2 and float(kv["Z_MAX"]) == 2e-4
+    assert nrs == {128, 256, 512} and nts == {512, 1024, 2048}
+    assert 0.5e-6 <= min(es) and max(es) <= 10e-6 and max(es) / min(es) > 5
+    assert 40e-6 <= min(ws) and max(ws) <= 150e-6 and max(ws) - min(ws) > 50e-6
