@@ -325,10 +325,12 @@ def test_chunked_march_equals_single_call(tmp_path):
     assert f.shape[0] == 4 and all(np.abs(f[k]).max() > 0 for k in range(4))
 
 
-def test_march_many_matches_single_runs(tmp_path):
+@pytest.mark.parametrize("prec", ["mixed", "fp32", "fp64", "each"])
+def test_march_many_matches_single_runs(tmp_path, prec):
     """gs_march_many over runs with uneven grids (N_R, N_T) and pulse energies,
-    mixed precision: every run's dumps and final field equal those of the same
-    run marched alone."""
+    in one precision or ("each") one run per precision in the same batch:
+    every run's dumps and final field equal those of the same run marched
+    alone."""
     specs = [dict(N_R="64", N_T="512", ENERGY="8e-6"), dict(N_R="128", N_T="1024", ENERGY="4e-6"),
              dict(N_R="64", N_T="256", ENERGY="12e-6")]
     cfgs = []
@@ -337,16 +339,17 @@ def test_march_many_matches_single_runs(tmp_path):
                   DUMP_FILENAME0="m_%s_f.udf", DUMPTYPE1="plasma_max", DUMP_FILENAME1="m_%s_pm.udf")
         kw.update(sp)
         cfgs.append(write_cfg(tmp_path / ("c%d.txt" % i), **kw))
+    precs = ["mixed", "fp32", "fp64"] if prec == "each" else [prec] * len(cfgs)
     single = []
     for i, c in enumerate(cfgs):
         d = tmp_path / ("s%d" % i)
         d.mkdir()
-        p = gsa.Plan(c, precision="mixed")
+        p = gsa.Plan(c, precision=precs[i])
         assert p.march(str(d)) == 0
         single.append((p.get(), p.scalar("STEP_N"), oracle.load_udf(str(d / "m__f.udf"))[1],
                        oracle.load_udf(str(d / "m__pm.udf"))[1]))
         p.close()
-    plans = [gsa.Plan(c, precision="mixed") for c in cfgs]
+    plans = [gsa.Plan(c, precision=precs[i]) for i, c in enumerate(cfgs)]
     dirs = []
     for i in range(len(cfgs)):
         d = tmp_path / ("b%d" % i)
