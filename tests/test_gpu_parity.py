@@ -153,6 +153,30 @@ def test_fixed_steps_vs_oracle(tmp_path, prec, nr):
     assert e2 < {"fp64": 1e-11, "fp32": 2e-5, "mixed": 2e-5}[prec]
 
 
+@pytest.mark.parametrize("energy", ["12e-6", "1e-6"])
+@pytest.mark.parametrize("mode", ["kerr", "fwd", "kerr_inv", "all"])
+def test_mixed_mfma_line_ffts_vs_oracle(tmp_path, monkeypatch, mode, energy):
+    """The opt-in split-BF16 MFMA line FFTs of the mixed policy (GS_MIX_FFT:
+    the Kerr forward transform, every forward transform, and with them the
+    inverse transform of lines that do not ionize): two z-steps at
+    N_T = 4096 against the FP64 oracle.  On a field that does not ionize
+    (1 uJ) they stay within the mixed policy's 2e-5.  On one that ionizes
+    (12 uJ; the inverse falls back to FP32 on its hot lines) the split-BF16
+    transform's ~2^-16 relative error, multiplied by the strong nonlinear
+    increment of these steps, gives ~6e-5: within the 1e-4 field bar but 3x
+    the VALU transforms' error, one reason the VALU FFTs stay the default."""
+    monkeypatch.setenv("GS_MIX_FFT", mode)
+    p, o = pair(tmp_path, "mixed", **dict(HOT, N_T="4096", ENERGY=energy))
+    for k in range(2):
+        o.fixed_step(2e-6)
+        p.fixed_step(2e-6)
+    err = rel_l2(p.get(), o.get("field"))
+    e2 = rel_l2(p.get(gsa.B2), o.get("b2"))
+    print("mfma fft", mode, energy, err, e2)
+    tol = 1e-4 if energy == "12e-6" else 2e-5
+    assert err < tol and e2 < tol
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
 def test_fixed_steps_production_line_length(tmp_path, prec):
     """Two z-steps at the production line length N_T = 4096 (the line kernel's
