@@ -289,17 +289,23 @@ def test_device_driven_march_matches_host_controller(tmp_path, prec, monkeypatch
     """gs_march runs the reference controller on the device (step attempts
     replayed from a captured graph, host reads once per chunk of attempts);
     the host-driven loop (one synchronisation per attempt) must give the same
-    step sequence and the same dumps and field, bit for bit.  Also covers
-    max_steps stops in the middle of a chunk and a repeated march call."""
+    step sequence and the same dumps and field, bit for bit, and so must the
+    device-driven attempts enqueued without the graph (GS_NO_GRAPH).  Also
+    covers max_steps stops in the middle of a chunk and a repeated march
+    call."""
     kw = dict(HOT, N_Z="3", Z_MAX="1e-4", MAX_TOLERANCE="3e-3", DUMPS_N="2", DUMPTYPE0="flux",
               DUMP_FILENAME0="d_%s_f.udf", DUMPTYPE1="plasma_max", DUMP_FILENAME1="d_%s_pm.udf")
     cfg = write_cfg(tmp_path / "c.txt", **kw)
     out = {}
-    for mode in ("host", "device"):
+    for mode in ("host", "device", "nograph"):
         if mode == "host":
             monkeypatch.setenv("GS_HOST_MARCH", "1")
         else:
             monkeypatch.delenv("GS_HOST_MARCH", raising=False)
+        if mode == "nograph":
+            monkeypatch.setenv("GS_NO_GRAPH", "1")
+        else:
+            monkeypatch.delenv("GS_NO_GRAPH", raising=False)
         d = tmp_path / mode
         d.mkdir()
         p = gsa.Plan(cfg, precision=prec)
@@ -310,13 +316,14 @@ def test_device_driven_march_matches_host_controller(tmp_path, prec, monkeypatch
                          rej=p.scalar("n_reject"), z=p.scalar("CURRENT_Z"),
                          f=oracle.load_udf(str(d / "d__f.udf"))[1], pm=oracle.load_udf(str(d / "d__pm.udf"))[1])
         p.close()
-    h, g = out["host"], out["device"]
-    print("march modes", prec, h["steps"], h["rej"], g["steps"], g["rej"])
-    assert h["mid"][1] == g["mid"][1] and h["mid"][2] == g["mid"][2]
-    np.testing.assert_array_equal(h["mid"][0], g["mid"][0])
-    assert (h["steps"], h["rej"], h["z"]) == (g["steps"], g["rej"], g["z"])
-    for k in ("field", "b2", "f", "pm"):
-        np.testing.assert_array_equal(h[k], g[k])
+    h = out["host"]
+    for g in (out["device"], out["nograph"]):
+        print("march modes", prec, h["steps"], h["rej"], g["steps"], g["rej"])
+        assert h["mid"][1] == g["mid"][1] and h["mid"][2] == g["mid"][2]
+        np.testing.assert_array_equal(h["mid"][0], g["mid"][0])
+        assert (h["steps"], h["rej"], h["z"]) == (g["steps"], g["rej"], g["z"])
+        for k in ("field", "b2", "f", "pm"):
+            np.testing.assert_array_equal(h[k], g[k])
 
 
 def test_chunked_march_equals_single_call(tmp_path):
